@@ -1,0 +1,1465 @@
+// amcafe_device.hip -- HIP kernels of the CAFE time step for CDNA4 (gfx950) and
+// the internal launcher API declared in amcafe_dev.h.
+//
+// One step of one or many independent domains ("instances") runs as:
+//   k_temperature      TempField::AnalyticTempCurr (S/TempField.C:139-217), optionally
+//                      fused with SetLiquid3 (S/VoxelsCA.C:1098-1116)
+//   k_setliquid        SetLiquid3 alone, when the temperature was a separate call
+//   k_solid_to_mushy   ConvertSolid1(1) (S/VoxelsCA.C:780-797)
+//   k_active_flags     active-set membership (S/VoxelsCA.C:262-273) + per-block counts
+//   k_scan             per-instance exclusive scan of the block counts
+//   k_compact          ordered active list, voxel->active map, gathers, ruq draws
+//                      (S/VoxelsCA.C:274-303, 336-380)
+//   k_active_setup     active-neighbour table (S/VoxelsCA.C:304-335), growing list and
+//                      capture candidates (the per-pair part of S/VoxelsCA.C:386-445)
+//   k_event_loop       the serial capture/nucleation loop (S/VoxelsCA.C:381-610), one
+//                      workgroup per instance, FP64 throughout
+//   k_scatter          active arrays back to the grid (S/VoxelsCA.C:633-643)
+//   k_mushy_to_solid   ConvertSolid1(0) over the active cells (S/VoxelsCA.C:739-758)
+// Grid-wide kernels take a flattened block list over all instances, so a batch of
+// domains of different sizes is one launch per stage.
+//
+// Must be compiled with -ffp-contract=off: every floating-point expression that
+// restates the reference is evaluated with its own rounding, in its own order.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "amcafe_dev.h"
+#include "ddmath.h"
+
+#define ZB 1024     // voxels per compaction block (one per thread)
+#define GB 256      // threads per grid-wide block
+#define EL_NT 256   // threads of the event-loop workgroup
+#define EL_NW (EL_NT / 64)
+
+// -------------------------------------------------------------------------------------
+// device structures
+struct LoopState {
+  int nslots;        // growing-list length
+  int vSind2;        // next scan position (S/VoxelsCA.C:346-353, 487-488, 520-521)
+  int nGrain;
+  int pool_used;
+  unsigned int rng;  // minstd_rand0 state
+  int pad0;
+  long long draws;
+  double tinc;
+  int events, captures, nucs, done;
+  int pending;       // 1: a nucleation was decided but the pool was empty
+  int pend_tgt;      // active index of the voxel to nucleate
+  int Na, nmushy0;
+};
+
+struct Inst {
+  amcd_params p;
+  int N, nxy, nblkZ, nblkG;
+  // grid fields
+  uint8_t* vs;
+  int* gid;
+  double* ext;
+  double *cx, *cy, *cz;
+  double* T;
+  uint8_t* bj;
+  int* amap;
+  int *bcA, *bcM;  // per-ZB-block counts -> exclusive offsets
+  // active list
+  int acap;
+  int* aVox;
+  uint8_t* aVS;
+  int* aG;
+  double *aT, *aExt, *aCx, *aCy, *aCz, *ruq;
+  int* aNbr;
+  uint8_t* aCold;
+  int* aScan;
+  // growing list + candidates (26 slots per growing cell)
+  int *gAct, *gScan;
+  double *gExt, *gVhat;
+  uint8_t *gN, *gDead, *gGrow;
+  double *cD, *cF, *cR;
+  int* cTgt;
+  uint8_t* cK;
+  // grains
+  int gcap;
+  double* rot;
+  int poolcap;
+  double* pool;
+  LoopState* ls;
+};
+
+// -------------------------------------------------------------------------------------
+// minstd_rand0 (x <- 16807 x mod 2^31-1) and libstdc++'s generate_canonical<double,53>
+// (/usr/include/c++/11/bits/random.tcc:3345-3381): two engine outputs per double,
+//   sum = (x1-1) + (x2-1)*R,  R = 2^31-2,  result = sum / RN(R*R),  clamp below 1.
+#define MINSTD_M 2147483647u
+__host__ __device__ static inline unsigned int mod_m(unsigned long long x) {
+  unsigned long long r = (x & MINSTD_M) + (x >> 31);
+  r = (r & MINSTD_M) + (r >> 31);
+  return (unsigned int)(r >= MINSTD_M ? r - MINSTD_M : r);
+}
+__host__ __device__ static inline unsigned int lcg_next(unsigned int x) { return mod_m((unsigned long long)x * 16807u); }
+__host__ __device__ static inline unsigned int lcg_pow(unsigned long long e) {  // 16807^e mod m
+  unsigned long long b = 16807u, r = 1;
+  while (e) {
+    if (e & 1) r = mod_m(r * b);
+    b = mod_m(b * b);
+    e >>= 1;
+  }
+  return (unsigned int)r;
+}
+__host__ __device__ static inline unsigned int lcg_seed(unsigned long long s) {
+  unsigned int x = (unsigned int)(s % MINSTD_M);
+  return x == 0 ? 1u : x;
+}
+__host__ __device__ static inline double canonical(unsigned int x1, unsigned int x2) {
+  double sum = (double)(x1 - 1u);
+  sum = sum + (double)(x2 - 1u) * 2147483646.0;
+  double ret = sum / 4611686009837453312.0;  // RN((2^31-2)^2)
+  if (ret >= 1.0) ret = 0x1.fffffffffffffp-1;
+  return ret;
+}
+
+// -------------------------------------------------------------------------------------
+__device__ static inline int find_inst(const int* start, int n, int b) {
+  int lo = 0, hi = n;  // start[lo] <= b < start[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+__device__ static inline void decode(const Inst& d, int v, int& i, int& j, int& k) {
+  k = v / d.nxy;
+  int r = v - k * d.nxy;
+  j = r / d.p.nx;
+  i = r - j * d.p.nx;
+}
+
+// -------------------------------------------------------------------------------------
+// TempField::AnalyticTempCurr, S/TempField.C:155-197, with the optional SetLiquid3.
+__global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, int ninst, const amcd_step* S,
+                                                    amcd_out* O, int setliquid) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  const amcd_step& st = S[ii];
+  if (st.skip) return;
+  int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
+  bool hot = false;
+  if (v < d.N) {
+    int i, j, k;
+    decode(d, v, i, j, k);
+    const amcd_params& p = d.p;
+    double x0 = ((double)i + .5) * p.dx;
+    double y0 = ((double)j + .5) * p.dy;
+    double zp = ((double)k + .5) * p.dz;
+    double T = p.T0;
+    if (!(zp > st.lz)) {
+      double xp = st.cg * (x0 - p.hx) + st.sg * (y0 - p.hy) + p.hx;
+      double yp = (-st.sg) * (x0 - p.hx) + st.cg * (y0 - p.hy) + p.hy;
+      double r0 = xp - st.lx, r1 = yp - st.ly, r2 = zp - st.lz;
+      if (st.dirp * r0 > 0) {
+        double q1 = r1 / p.a1[1], q2 = r2 / p.a1[2];
+        double dsq = q1 * q1 + q2 * q2;
+        T = (dsq < 1.0 && (fabs(r0) < p.bmDX0)) ? p.xi : p.tS;
+      } else {
+        double q0 = r0 / p.a1[3], q1 = r1 / p.a1[4], q2 = r2 / p.a1[5];
+        double dsq = q0 * q0 + q1 * q1 + q2 * q2;
+        T = (dsq < 1.0) ? p.xi : p.tS;
+      }
+      if (T > p.tL) d.bj[v] = 1;
+      else if (T < p.T0) d.bj[v] = 0;
+    }
+    d.T[v] = T;
+    hot = T >= p.tL;
+    uint8_t s = d.vs[v] & 3;
+    if (setliquid && hot) {
+      if (v < d.nxy) {
+        s = 2;
+      } else if (v < st.ilaserLoc * d.nxy) {
+        s = 1;
+        d.gid[v] = 0;
+      }
+      d.ext[v] = 0.0;
+    }
+    d.vs[v] = s | (hot ? 0x80 : 0);
+  }
+  if (__any(hot) && (threadIdx.x & 63) == 0) atomicOr(&O[ii].any_melt, 1);
+}
+
+// SetLiquid3 (S/VoxelsCA.C:1098-1116) when the temperature kernel ran without it.
+__global__ __launch_bounds__(GB) void k_setliquid(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (S[ii].skip) return;
+  int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
+  if (v >= d.N) return;
+  uint8_t s = d.vs[v];
+  if (!(s & 0x80)) return;
+  uint8_t st = s & 3;
+  if (v < d.nxy) {
+    st = 2;
+  } else if (v < S[ii].ilaserLoc * d.nxy) {
+    st = 1;
+    d.gid[v] = 0;
+  }
+  d.ext[v] = 0.0;
+  d.vs[v] = st | 0x80;
+}
+
+// ConvertSolid1(1): solid -> mushy when any Moore neighbour is liquid.  Updating in
+// place is equivalent to the reference's collect-then-apply, because the update
+// (3 -> 2) never changes the predicate (some neighbour == 1) of any other voxel.
+__global__ __launch_bounds__(GB) void k_solid_to_mushy(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (S[ii].skip) return;
+  int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
+  if (v >= d.N) return;
+  uint8_t s = d.vs[v];
+  if ((s & 3) != 3) return;
+  int i, j, k;
+  decode(d, v, i, j, k);
+  const int nx = d.p.nx, ny = d.p.ny, nz = d.p.nz;
+  for (int dz = -1; dz <= 1; ++dz) {
+    int z = k + dz;
+    if (z < 0 || z >= nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int y = j + dy;
+      if (y < 0 || y >= ny) continue;
+      const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * nx;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int x = i + dx;
+        if (x < 0 || x >= nx || (dx == 0 && dy == 0 && dz == 0)) continue;
+        if ((row[x] & 3) == 1) {
+          d.vs[v] = (s & ~3) | 2;
+          return;
+        }
+      }
+    }
+  }
+}
+
+// Active set: {mushy} U {liquid below tL} U {neighbours at or above tL of those}.
+// Writes bit 6 of vs and the per-block (ZB voxels) counts of active and mushy cells.
+__global__ __launch_bounds__(ZB) void k_active_flags(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
+  __shared__ int sA[ZB / 64], sM[ZB / 64];
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (S[ii].skip) return;
+  int blk = blockIdx.x - bstart[ii];
+  int v = blk * ZB + threadIdx.x;
+  bool act = false, mushy = false;
+  if (v < d.N) {
+    uint8_t s = d.vs[v];
+    int st = s & 3;
+    bool hot = (s & 0x80) != 0;
+    mushy = st == 2;
+    act = mushy || (st == 1 && !hot);
+    if (!act && hot) {
+      int i, j, k;
+      decode(d, v, i, j, k);
+      for (int dz = -1; dz <= 1 && !act; ++dz) {
+        int z = k + dz;
+        if (z < 0 || z >= d.p.nz) continue;
+        for (int dy = -1; dy <= 1 && !act; ++dy) {
+          int y = j + dy;
+          if (y < 0 || y >= d.p.ny) continue;
+          const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * d.p.nx;
+          for (int dx = -1; dx <= 1; ++dx) {
+            int x = i + dx;
+            if (x < 0 || x >= d.p.nx || (dx == 0 && dy == 0 && dz == 0)) continue;
+            uint8_t sn = row[x];
+            int stn = sn & 3;
+            if (stn == 2 || (stn == 1 && !(sn & 0x80))) {
+              act = true;
+              break;
+            }
+          }
+        }
+      }
+    }
+    d.vs[v] = (s & ~0x40) | (act ? 0x40 : 0);
+  }
+  unsigned long long ba = __ballot(act), bm = __ballot(mushy);
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sA[w] = __popcll(ba);
+    sM[w] = __popcll(bm);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a = 0, m = 0;
+    for (int q = 0; q < ZB / 64; ++q) {
+      a += sA[q];
+      m += sM[q];
+    }
+    d.bcA[blk] = a;
+    d.bcM[blk] = m;
+  }
+}
+
+// Exclusive scan of the block counts, one workgroup per instance; sizes the step.
+__global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd_out* O) {
+  __shared__ int sA[1024], sM[1024];
+  const Inst& d = I[blockIdx.x];
+  int nb = d.nblkZ, t = threadIdx.x;
+  int per = (nb + 1023) / 1024;
+  int b0 = t * per, b1 = min(nb, b0 + per);
+  int a = 0, m = 0;
+  for (int b = b0; b < b1; ++b) {
+    a += d.bcA[b];
+    m += d.bcM[b];
+  }
+  sA[t] = a;
+  sM[t] = m;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+    int xa = t >= off ? sA[t - off] : 0, xm = t >= off ? sM[t - off] : 0;
+    __syncthreads();
+    sA[t] += xa;
+    sM[t] += xm;
+    __syncthreads();
+  }
+  int ra = sA[t] - a, rm = sM[t] - m;
+  for (int b = b0; b < b1; ++b) {
+    int ca = d.bcA[b], cm = d.bcM[b];
+    d.bcA[b] = ra;
+    d.bcM[b] = rm;
+    ra += ca;
+    rm += cm;
+  }
+  if (t == 1023) {
+    int Na = sA[1023], Nm = sM[1023];
+    if (S[blockIdx.x].skip) Na = Nm = 0;
+    LoopState* ls = d.ls;
+    ls->Na = Na;
+    ls->nmushy0 = Nm;
+    ls->nslots = 0;
+    ls->vSind2 = Nm;
+    ls->tinc = 0.0;
+    ls->events = ls->captures = ls->nucs = 0;
+    ls->pool_used = 0;
+    ls->pending = 0;
+    ls->done = 0;
+    ls->nGrain = S[blockIdx.x].nGrain;
+    amcd_out& o = O[blockIdx.x];
+    o.n_active = Na;
+    o.n_mushy0 = Nm;
+    if (Na > d.acap) {
+      o.overflow = 1;
+      o.status = -2;
+      ls->done = 1;
+    }
+    if (S[blockIdx.x].skip) ls->done = 1;
+  }
+}
+
+// Ordered compaction: the active list in ascending voxel id, the voxel -> active map,
+// gathers of the per-cell state, the scan position of mushy cells and the uniform
+// draws ruq (S/VoxelsCA.C:380): the engine is seeded with 30*tInd+seed1 and active
+// cell a takes engine outputs 2a+1 and 2a+2 (jump-ahead 16807^(2a)).
+__global__ __launch_bounds__(ZB) void k_compact(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
+  __shared__ int sA[ZB / 64], sM[ZB / 64];
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (d.ls->done) return;
+  int blk = blockIdx.x - bstart[ii];
+  int v = blk * ZB + threadIdx.x;
+  bool act = false, mushy = false;
+  uint8_t s = 0;
+  if (v < d.N) {
+    s = d.vs[v];
+    act = (s & 0x40) != 0;
+    mushy = (s & 3) == 2;
+  }
+  unsigned long long ba = __ballot(act), bm = __ballot(mushy);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (lane == 0) {
+    sA[w] = __popcll(ba);
+    sM[w] = __popcll(bm);
+  }
+  __syncthreads();
+  if (!act) return;
+  int offA = d.bcA[blk], offM = d.bcM[blk];
+  for (int q = 0; q < w; ++q) {
+    offA += sA[q];
+    offM += sM[q];
+  }
+  int a = offA + __popcll(ba & lt);
+  d.aVox[a] = v;
+  d.amap[v] = a;
+  d.aVS[a] = s & 3;
+  d.aG[a] = d.gid[v];
+  d.aT[a] = d.T[v];
+  d.aExt[a] = d.ext[v];
+  d.aCx[a] = d.cx[v];
+  d.aCy[a] = d.cy[v];
+  d.aCz[a] = d.cz[v];
+  d.aScan[a] = mushy ? offM + __popcll(bm & lt) : -1;
+  unsigned int x0 = lcg_seed(30ull * (unsigned long long)S[ii].tInd + d.p.seed1);
+  unsigned int x1 = mod_m((unsigned long long)x0 * lcg_pow(2ull * a + 1));
+  unsigned int x2 = lcg_next(x1);
+  d.ruq[a] = (canonical(x1, x2) - .5) * d.p.dx / 100.;
+}
+
+// Voxel centre, S/VoxelsCA.C:409-414.
+__device__ static inline void centre(const Inst& d, int v, double* c) {
+  int i, j, k;
+  decode(d, v, i, j, k);
+  c[0] = ((double)i + .5) * d.p.dx;
+  c[1] = ((double)j + .5) * d.p.dy;
+  c[2] = ((double)k + .5) * d.p.dz;
+}
+
+// Capture candidate of growing cell a (centroid c, rotation R) towards liquid voxel
+// vt: D = pow(cos th sin ph,.5) * |R^T dnx|_1 and the factor f (S/VoxelsCA.C:409-424).
+__device__ static inline void candidate(const Inst& d, const double* c, const double* R, int vt, double* D,
+                                        double* f) {
+  double ct[3];
+  centre(d, vt, ct);
+  double dnx0 = ct[0] - c[0], dnx1 = ct[1] - c[1], dnx2 = ct[2] - c[2];
+  double ff = amc_shape_factor(dnx0, dnx1, dnx2);
+  double d0 = fabs(R[0] * dnx0 + R[3] * dnx1 + R[6] * dnx2);
+  double d1 = fabs(R[1] * dnx0 + R[4] * dnx1 + R[7] * dnx2);
+  double d2 = fabs(R[2] * dnx0 + R[5] * dnx1 + R[8] * dnx2);
+  *D = ff * (d0 + d1 + d2);
+  *f = ff;
+}
+
+__device__ static inline double growth_velocity(const amcd_params& p, double T, int* err) {
+  if (T >= p.tL) return 0.0;  // S/VoxelsCA.C:405
+  if (T == p.tS) return p.vhat_tS;
+  if (T == p.T0) return p.vhat_T0;
+  *err = 1;
+  return __builtin_nan("");
+}
+
+// Active-neighbour table and, for every growing cell (mushy, some liquid active
+// neighbour, all active neighbours below tL; S/VoxelsCA.C:396-397), its slot in the
+// growing list and its capture candidates.  Slot order is irrelevant: the event loop
+// compares candidates by (time, ruqs, scan position, neighbour rank).
+__global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart, int ninst, amcd_out* O) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (d.ls->done) return;
+  int a = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
+  int Na = d.ls->Na;
+  bool grow = false;
+  if (a < Na) {
+    int v = d.aVox[a];
+    int i, j, k;
+    decode(d, v, i, j, k);
+    int* nb = d.aNbr + (size_t)a * AMCD_NB;
+    int q = 0;
+    bool cold = true, liq = false;
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          if (dx == 0 && dy == 0 && dz == 0) continue;
+          int x = i + dx, y = j + dy, z = k + dz;
+          int n = -1;
+          if (x >= 0 && x < d.p.nx && y >= 0 && y < d.p.ny && z >= 0 && z < d.p.nz)
+            n = d.amap[v + dz * d.nxy + dy * d.p.nx + dx];
+          nb[q++] = n;
+          if (n >= 0) {
+            if (!(d.aT[n] < d.p.tL)) cold = false;
+            if (d.aVS[n] == 1) liq = true;
+          }
+        }
+    d.aCold[a] = cold;
+    grow = cold && liq && d.aVS[a] == 2;
+  }
+  unsigned long long bal = __ballot(grow);
+  if (!bal) return;
+  int lane = threadIdx.x & 63;
+  int leader = __ffsll((long long)bal) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(&d.ls->nslots, __popcll(bal));
+  base = __shfl(base, leader);
+  if (!grow) return;
+  unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int slot = base + __popcll(bal & lt);
+  int err = 0;
+  d.gAct[slot] = a;
+  d.gScan[slot] = d.aScan[a];
+  d.gExt[slot] = d.aExt[a];
+  d.gVhat[slot] = growth_velocity(d.p, d.aT[a], &err);
+  d.gDead[slot] = 0;
+  if (err) atomicOr(&O[ii].err_nonanalytic, 1);
+  double c[3] = {d.aCx[a], d.aCy[a], d.aCz[a]};
+  const int G = d.aG[a];
+  if (G < 1 || G > d.gcap) {  // a growing cell must carry a valid grain
+    atomicOr(&O[ii].err_nonanalytic, 2);
+    d.gN[slot] = 0;
+    d.gDead[slot] = 1;
+    return;
+  }
+  const double* R = d.rot + 9 * (size_t)(G - 1);
+  double Rl[9];
+  for (int q = 0; q < 9; ++q) Rl[q] = R[q];
+  const int* nb = d.aNbr + (size_t)a * AMCD_NB;
+  int n = 0;
+  size_t cb = (size_t)slot * AMCD_NB;
+  for (int q = 0; q < AMCD_NB; ++q) {
+    int t = nb[q];
+    if (t < 0 || d.aVS[t] != 1) continue;
+    double D, f;
+    candidate(d, c, Rl, d.aVox[t], &D, &f);
+    d.cD[cb + n] = D;
+    d.cF[cb + n] = f;
+    d.cR[cb + n] = .5 * (d.ruq[a] + d.ruq[t]);
+    d.cTgt[cb + n] = t;
+    d.cK[cb + n] = (uint8_t)q;
+    ++n;
+  }
+  d.gN[slot] = (uint8_t)n;
+}
+
+// Re-centring of the decentred octahedron after a capture, S/VoxelsCA.C:528-605 and
+// the helpers S/VoxelsCA.h:56-93.  R is row-major (local -> global), f is the
+// capturing candidate's shape factor.  Outputs the new centroid and half-diagonal.
+__device__ static inline void project_point_line(const double* A, const double* x0, const double* x1, double* xp) {
+  double n[3], snorm, dd, t;
+  snorm = sqrt((x1[0] - x0[0]) * (x1[0] - x0[0]) + (x1[1] - x0[1]) * (x1[1] - x0[1]) +
+               (x1[2] - x0[2]) * (x1[2] - x0[2]));
+  n[0] = (x1[0] - x0[0]) / snorm;
+  n[1] = (x1[1] - x0[1]) / snorm;
+  n[2] = (x1[2] - x0[2]) / snorm;
+  dd = -(n[0] * A[0] + n[1] * A[1] + n[2] * A[2]);
+  t = (-dd - n[0] * x0[0] - n[1] * x0[1] - n[2] * x0[2]) /
+      (n[0] * (x1[0] - x0[0]) + n[1] * (x1[1] - x0[1]) + n[2] * (x1[2] - x0[2]));
+  xp[0] = x0[0] + (x1[0] - x0[0]) * t;
+  xp[1] = x0[1] + (x1[1] - x0[1]) * t;
+  xp[2] = x0[2] + (x1[2] - x0[2]) * t;
+}
+__device__ static inline double dist3(const double* p, const double* q) {
+  return sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]));
+}
+__device__ static void recentre(const Inst& d, int vcap, int vnew, const double* cen, const double* R, double f,
+                                double* ncen, double* next) {
+  const amcd_params& p = d.p;
+  int jx0, jx1, jx2, jy0, jy1, jy2;
+  decode(d, vnew, jx0, jx1, jx2);
+  decode(d, vcap, jy0, jy1, jy2);
+  double l0 = (jx0 - jy0) * p.dx, l1 = (jx1 - jy1) * p.dy, l2 = (jx2 - jy2) * p.dz;
+  double l = sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+  double dnx[3], locX[3];
+  dnx[0] = ((double)jx0 + .5) * p.dx - cen[0];
+  dnx[1] = ((double)jx1 + .5) * p.dy - cen[1];
+  dnx[2] = ((double)jx2 + .5) * p.dz - cen[2];
+  locX[0] = R[0] * dnx[0] + R[3] * dnx[1] + R[6] * dnx[2];
+  locX[1] = R[1] * dnx[0] + R[4] * dnx[1] + R[7] * dnx[2];
+  locX[2] = R[2] * dnx[0] + R[5] * dnx[1] + R[8] * dnx[2];
+  double dr = fabs(locX[0]) + fabs(locX[1]) + fabs(locX[2]);
+  // octahedron vertices +-dr e_i; the octant's three vertices (S/VoxelsCA.h:61-76)
+  const int sInd[8][3] = {{0, 1, 2}, {1, 2, 3}, {0, 2, 4}, {2, 3, 4}, {0, 1, 5}, {1, 3, 5}, {0, 4, 5}, {3, 4, 5}};
+  int jInd = 4 * (signbit(locX[2]) ? 1 : 0) + 2 * (signbit(locX[1]) ? 1 : 0) + (signbit(locX[0]) ? 1 : 0);
+  double V[3][3], S0[3][3];
+  for (int q = 0; q < 3; ++q) {
+    int vi = sInd[jInd][q];
+    for (int c = 0; c < 3; ++c) {
+      double sgn = (vi % 3 == c) ? (vi < 3 ? 1.0 : -1.0) : 0.0;
+      S0[q][c] = sgn;
+      V[q][c] = sgn * dr;
+    }
+  }
+  double dn[3];
+  for (int q = 0; q < 3; ++q)
+    dn[q] = (locX[0] - V[q][0]) * (locX[0] - V[q][0]) + (locX[1] - V[q][1]) * (locX[1] - V[q][1]) +
+            (locX[2] - V[q][2]) * (locX[2] - V[q][2]);
+  int k0 = 0, k2 = 0;
+  for (int q = 1; q < 3; ++q) {
+    if (dn[q] < dn[k0]) k0 = q;   // std::min_element: first minimum
+    if (dn[k2] < dn[q]) k2 = q;   // std::max_element: first maximum
+  }
+  int k1;
+  if (k0 == k2) {
+    k0 = 0; k1 = 1; k2 = 2;
+  } else {
+    k1 = 3 - k0 - k2;
+  }
+  double xI[3], xJ[3];
+  project_point_line(locX, V[k0], V[k1], xI);
+  project_point_line(locX, V[k0], V[k2], xJ);
+  double d1I = dist3(V[k0], xI), dI2 = dist3(V[k1], xI), d1J = dist3(V[k0], xJ), dJ3 = dist3(V[k2], xJ);
+  const double sq3 = 0x1.bb67ae8584caap+0;   // pow(3.0,.5), folded by the reference's compiler
+  const double s23 = 0x1.a20bd700c2c3ep-1;   // pow(2.0/3.0,.5)
+  double m = sq3 * l;
+  double L12 = .5 * ((m < d1I ? m : d1I) + (m < dI2 ? m : dI2));
+  double L13 = .5 * ((m < d1J ? m : d1J) + (m < dJ3 ? m : dJ3));
+  double Lmud = sq3 * (s23 * (L12 < L13 ? L13 : L12));
+  *next = f * Lmud * 1.0;
+  dnx[0] = V[k0][0] - Lmud * S0[k0][0];
+  dnx[1] = V[k0][1] - Lmud * S0[k0][1];
+  dnx[2] = V[k0][2] - Lmud * S0[k0][2];
+  locX[0] = R[0] * dnx[0] + R[1] * dnx[1] + R[2] * dnx[2];
+  locX[1] = R[3] * dnx[0] + R[4] * dnx[1] + R[5] * dnx[2];
+  locX[2] = R[6] * dnx[0] + R[7] * dnx[1] + R[8] * dnx[2];
+  ncen[0] = cen[0] + locX[0];
+  ncen[1] = cen[1] + locX[1];
+  ncen[2] = cen[2] + locX[2];
+}
+
+struct Best {
+  double t, r;
+  long long key;
+  int cand;
+};
+__device__ static inline bool better(const Best& a, const Best& b) {  // a strictly before b
+  if (a.t < b.t) return true;
+  if (a.t > b.t) return false;
+  if (a.r < b.r) return true;
+  if (a.r > b.r) return false;
+  return a.key < b.key;
+}
+__device__ static inline Best shfl_best(const Best& b, int lane_mask) {
+  Best o;
+  o.t = __shfl_xor(b.t, lane_mask);
+  o.r = __shfl_xor(b.r, lane_mask);
+  o.key = __shfl_xor(b.key, lane_mask);
+  o.cand = __shfl_xor(b.cand, lane_mask);
+  return o;
+}
+
+// The capture / nucleation loop of one instance (S/VoxelsCA.C:381-610).
+// Each iteration: every live candidate's time t = (D - ExtA)/vhat, the lexicographic
+// minimum of (t, ruqs, scan order) over all of them, one uniform draw, then either a
+// nucleation at the captured voxel or growth of every growing cell by vhat*Dt and a
+// capture with octahedron re-centring.  The workgroup keeps going while tinc < DelT.
+__global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* S, amcd_out* O) {
+  __shared__ Best sb[EL_NW];
+  __shared__ int s_go, s_nuc, s_tgt, s_slot, s_cand, s_nslots, s_newslot;
+  __shared__ double s_dt;
+  const Inst& d = I[blockIdx.x];
+  LoopState* ls = d.ls;
+  if (ls->done) return;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const amcd_params& p = d.p;
+  const amcd_step& st = S[blockIdx.x];
+  // thread 0 owns the scalar loop state
+  unsigned int rng = 0;
+  double tinc = 0;
+  int vSind2 = 0, nGrain = 0, pool_used = 0, events = 0, captures = 0, nucs = 0;
+  long long draws = 0;
+  if (tid == 0) {
+    rng = ls->rng;
+    tinc = ls->tinc;
+    vSind2 = ls->vSind2;
+    nGrain = ls->nGrain;
+    pool_used = ls->pool_used;
+    events = ls->events;
+    captures = ls->captures;
+    nucs = ls->nucs;
+    draws = ls->draws;
+    s_nslots = ls->nslots;
+    s_go = 1;
+    if (ls->pending) {  // resume: apply the nucleation decided before the pool ran out
+      s_nuc = 1;
+      s_tgt = ls->pend_tgt;
+      s_dt = 0.0;
+      s_go = 2;
+    } else if (events == 0 && ls->Na == ls->nmushy0) {
+      s_go = 0;  // loop condition vSind2 != countS fails at once (S/VoxelsCA.C:381)
+    }
+  }
+  __syncthreads();
+  while (s_go) {
+    if (s_go == 1) {
+      // ---- scan: every thread takes slots tid, tid+NT, ...
+      Best b;
+      b.t = __builtin_inf();
+      b.r = __builtin_inf();
+      b.key = 0x7fffffffffffffffll;
+      b.cand = -1;
+      const int ns = s_nslots;
+      for (int s = tid; s < ns; s += EL_NT) {
+        if (d.gDead[s]) continue;
+        const double E = d.gExt[s], V = d.gVhat[s];
+        const int n = d.gN[s];
+        const size_t cb = (size_t)s * AMCD_NB;
+        const long long kb = (long long)d.gScan[s] * 32;
+        bool live = false;
+        for (int q = 0; q < n; ++q) {
+          int tg = d.cTgt[cb + q];
+          if (d.aVS[tg] != 1) continue;
+          live = true;
+          double dr = d.cD[cb + q] - E;
+          double t = dr / V;
+          if (!(t <= 1e6)) continue;
+          Best c;
+          c.t = t;
+          c.r = d.cR[cb + q];
+          c.key = kb + d.cK[cb + q];
+          c.cand = (int)(cb + q);
+          if (better(c, b)) b = c;
+        }
+        d.gGrow[s] = live;
+        if (!live) d.gDead[s] = 1;
+      }
+      for (int m = 32; m >= 1; m >>= 1) {
+        Best o = shfl_best(b, m);
+        if (better(o, b)) b = o;
+      }
+      if (lane == 0) sb[w] = b;
+      __syncthreads();
+      if (tid == 0) {
+        Best bb = sb[0];
+        for (int q = 1; q < EL_NW; ++q)
+          if (better(sb[q], bb)) bb = sb[q];
+        if (bb.cand < 0 || bb.t >= 1e6) {
+          s_go = 0;  // S/VoxelsCA.C:480
+        } else {
+          unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
+          rng = x2;
+          draws += 2;
+          double U = canonical(x1, x2);
+          s_cand = bb.cand;
+          s_slot = bb.cand / AMCD_NB;
+          s_tgt = d.cTgt[bb.cand];
+          s_nuc = U < p.rX;
+          s_dt = s_nuc ? 0.0 : bb.t;
+          events++;
+        }
+      }
+      __syncthreads();
+      if (!s_go) break;
+      if (!s_nuc) {
+        // ---- growth of every cell that was growing at this scan (S/VoxelsCA.C:511-516)
+        const double Dt = s_dt;
+        for (int s = tid; s < ns; s += EL_NT) {
+          if (!d.gGrow[s]) continue;
+          double e = d.gExt[s] + d.gVhat[s] * Dt;
+          d.gExt[s] = (e < 0.0) ? 0.0 : e;
+        }
+      }
+    }
+    // ---- capture or nucleation of voxel s_tgt (thread 0)
+    if (tid == 0) {
+      const int i1 = s_tgt;
+      const int v1 = d.aVox[i1];
+      s_newslot = 0;
+      if (s_nuc) {
+        if (pool_used >= st.n_pool || pool_used >= d.poolcap || nGrain >= d.gcap) {  // suspend; host refills
+          ls->pending = 1;
+          ls->pend_tgt = i1;
+          s_go = -1;
+        } else {
+          const double* P = d.pool + 9 * (size_t)pool_used;
+          nGrain += 1;
+          double* R = d.rot + 9 * (size_t)(nGrain - 1);
+          for (int q = 0; q < 9; ++q) R[q] = P[q];
+          pool_used++;
+          nucs++;
+          double c[3];
+          centre(d, v1, c);
+          d.aG[i1] = nGrain;
+          d.aCx[i1] = c[0];
+          d.aCy[i1] = c[1];
+          d.aCz[i1] = c[2];
+          d.aExt[i1] = 0.0;
+          ls->pending = 0;
+        }
+      } else {
+        const int js = d.gAct[s_slot];
+        const int G = min(max(d.aG[js], 1), d.gcap);  // valid by construction (checked at slot creation)
+        const double* R = d.rot + 9 * (size_t)(G - 1);
+        double cen[3] = {d.aCx[js], d.aCy[js], d.aCz[js]}, nc[3], ne;
+        recentre(d, d.aVox[js], v1, cen, R, d.cF[s_cand], nc, &ne);
+        d.aG[i1] = G;
+        d.aCx[i1] = nc[0];
+        d.aCy[i1] = nc[1];
+        d.aCz[i1] = nc[2];
+        d.aExt[i1] = ne;
+        captures++;
+      }
+      if (s_go != -1) {
+        d.aVS[i1] = 2;
+        d.aScan[i1] = vSind2;
+        vSind2++;
+        tinc += s_dt;
+        s_newslot = d.aCold[i1];
+      }
+    }
+    __syncthreads();
+    if (s_go == -1) break;
+    // ---- the new mushy voxel joins the growing list if it has liquid neighbours
+    if (s_newslot && w == 0) {
+      const int i1 = s_tgt;
+      int t = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
+      bool liq = t >= 0 && d.aVS[t] == 1;
+      unsigned long long bal = __ballot(liq);
+      if (bal) {
+        const int slot = s_nslots;
+        const size_t cb = (size_t)slot * AMCD_NB;
+        if (liq) {
+          unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+          int r = __popcll(bal & lt);
+          const int G = min(max(d.aG[i1], 1), d.gcap);
+          const double* Rg = d.rot + 9 * (size_t)(G - 1);
+          double Rl[9];
+          for (int q = 0; q < 9; ++q) Rl[q] = Rg[q];
+          double c[3] = {d.aCx[i1], d.aCy[i1], d.aCz[i1]};
+          double D, f;
+          candidate(d, c, Rl, d.aVox[t], &D, &f);
+          d.cD[cb + r] = D;
+          d.cF[cb + r] = f;
+          d.cR[cb + r] = .5 * (d.ruq[i1] + d.ruq[t]);
+          d.cTgt[cb + r] = t;
+          d.cK[cb + r] = (uint8_t)lane;
+        }
+        if (lane == 0) {
+          int err = 0;
+          d.gAct[slot] = i1;
+          d.gScan[slot] = d.aScan[i1];
+          d.gExt[slot] = d.aExt[i1];
+          d.gVhat[slot] = growth_velocity(p, d.aT[i1], &err);
+          d.gN[slot] = (uint8_t)__popcll(bal);
+          d.gDead[slot] = 0;
+          if (err) atomicOr(&O[blockIdx.x].err_nonanalytic, 1);
+          s_nslots = slot + 1;
+        }
+      }
+    }
+    if (tid == 0) {
+      if (!(tinc < p.DelT)) s_go = 0;  // loop condition, S/VoxelsCA.C:381
+      else s_go = 1;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // ---- write back the growing cells' half-diagonals and the loop state
+  const int ns = s_nslots;
+  const bool suspended = (s_go == -1);
+  if (!suspended)
+    for (int s = tid; s < ns; s += EL_NT) d.aExt[d.gAct[s]] = d.gExt[s];
+  if (tid == 0) {
+    ls->rng = rng;
+    ls->tinc = tinc;
+    ls->vSind2 = vSind2;
+    ls->nGrain = nGrain;
+    ls->pool_used = pool_used;
+    ls->events = events;
+    ls->captures = captures;
+    ls->nucs = nucs;
+    ls->draws = draws;
+    ls->nslots = ns;
+    ls->done = suspended ? 0 : 1;
+    amcd_out& o = O[blockIdx.x];
+    o.status = suspended ? 1 : 0;
+    o.n_events = events;
+    o.n_captures = captures;
+    o.n_nucleations = nucs;
+    o.n_slots = ns;
+    o.nGrain = nGrain;
+    o.pool_used = pool_used;
+    o.draws = draws;
+    o.tinc = tinc;
+  }
+}
+
+// Active arrays back to the grid (S/VoxelsCA.C:633-643).
+__global__ __launch_bounds__(GB) void k_scatter(Inst* I, const int* astart, int ninst) {
+  int ii = find_inst(astart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  int a = (blockIdx.x - astart[ii]) * GB + threadIdx.x;
+  if (d.ls->pending || a >= d.ls->Na || a >= d.acap) return;
+  int v = d.aVox[a];
+  d.vs[v] = (d.vs[v] & 0xC0) | d.aVS[a];
+  d.gid[v] = d.aG[a];
+  d.ext[v] = d.aExt[a];
+  d.cx[v] = d.aCx[a];
+  d.cy[v] = d.aCy[a];
+  d.cz[v] = d.aCz[a];
+}
+
+// ConvertSolid1(0) (S/VoxelsCA.C:739-758): mushy -> solid when every Moore neighbour
+// is mushy or solid.  Every mushy voxel is in the active list, so only those are
+// visited; in-place update is safe because 2 -> 3 keeps "neighbour >= 2" true.
+// Also clears the voxel -> active map for the next step.
+__global__ __launch_bounds__(GB) void k_mushy_to_solid(Inst* I, const int* astart, int ninst) {
+  int ii = find_inst(astart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  int a = (blockIdx.x - astart[ii]) * GB + threadIdx.x;
+  if (d.ls->pending || a >= d.ls->Na || a >= d.acap) return;
+  int v = d.aVox[a];
+  d.amap[v] = -1;
+  if (d.aVS[a] != 2) return;
+  int i, j, k;
+  decode(d, v, i, j, k);
+  for (int dz = -1; dz <= 1; ++dz) {
+    int z = k + dz;
+    if (z < 0 || z >= d.p.nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int y = j + dy;
+      if (y < 0 || y >= d.p.ny) continue;
+      const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * d.p.nx;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int x = i + dx;
+        if (x < 0 || x >= d.p.nx || (dx == 0 && dy == 0 && dz == 0)) continue;
+        if ((row[x] & 3) < 2) return;
+      }
+    }
+  }
+  d.vs[v] = (d.vs[v] & 0xFC) | 3;
+  d.ext[v] = 0.0;
+}
+
+// ---- per-layer helpers ---------------------------------------------------------------
+__global__ void k_grain_presence(const int* gid, int N, int* present, int ngrain) {
+  size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)N) return;
+  int g = gid[v];
+  if (g > 0 && g <= ngrain && present[g - 1] == 0) present[g - 1] = 1;
+}
+__global__ void k_remap(int* gid, int N, const int* remap, int ngrain) {
+  size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)N) return;
+  int g = gid[v];
+  if (g > 0 && g <= ngrain) gid[v] = remap[g - 1];
+}
+// AddLayer1's voxel writes (S/VoxelsCA.C:952-981): the new layer takes the host's
+// Voronoi labels, everything at or above the laser plane is emptied.
+__global__ void k_write_layer(Inst* I, int ii, int iz1, int ilaserLoc, int gbase, const int* gtmp) {
+  const Inst& d = I[ii];
+  size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)d.N) return;
+  int i, j, k;
+  decode(d, (int)v, i, j, k);
+  if (k >= iz1 && k < ilaserLoc) {
+    int jind = d.nxy * (k - iz1) + d.p.nx * j + i;
+    d.gid[v] = gbase + gtmp[jind];
+    d.vs[v] = 3;
+    d.cx[v] = ((double)i + .5) * d.p.dx;
+    d.cy[v] = ((double)j + .5) * d.p.dy;
+    d.cz[v] = ((double)k + .5) * d.p.dz;
+  }
+  if ((long long)v >= (long long)d.nxy * ilaserLoc) {
+    d.vs[v] = 0;
+    d.gid[v] = 0;
+  }
+}
+
+// -------------------------------------------------------------------------------------
+// host side of the device library
+struct amcd_batch {
+  int n = 0;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Inst> h;      // host mirror of the descriptors
+  Inst* d_inst = nullptr;
+  int *d_bstartG = nullptr, *d_bstartZ = nullptr, *d_astart = nullptr;
+  std::vector<int> bstartG, bstartZ, astart;
+  amcd_step* d_steps = nullptr;
+  amcd_out* d_outs = nullptr;
+  amcd_step* h_steps = nullptr;  // pinned
+  amcd_out* h_outs = nullptr;    // pinned
+  std::vector<void*> allocs;
+  hipEvent_t ev[16];
+  double last_ms[16] = {0};
+  bool timing = false;
+};
+
+static thread_local std::string g_dev_err;
+#define HIPT(x)                                                                        \
+  do {                                                                                 \
+    hipError_t e_ = (x);                                                               \
+    if (e_ != hipSuccess) {                                                            \
+      g_dev_err = std::string(#x) + ": " + hipGetErrorString(e_);                      \
+      return -1;                                                                       \
+    }                                                                                  \
+  } while (0)
+
+template <class T>
+static int dalloc(amcd_batch* b, T** p, size_t n) {
+  void* q = nullptr;
+  HIPT(hipMalloc(&q, n * sizeof(T) + 256));
+  b->allocs.push_back(q);
+  *p = (T*)q;
+  return 0;
+}
+
+static int alloc_active(amcd_batch* b, Inst& d, int acap) {
+  d.acap = acap;
+  size_t c = (size_t)acap;
+  if (dalloc(b, &d.aVox, c) || dalloc(b, &d.aVS, c) || dalloc(b, &d.aG, c) || dalloc(b, &d.aT, c) ||
+      dalloc(b, &d.aExt, c) || dalloc(b, &d.aCx, c) || dalloc(b, &d.aCy, c) || dalloc(b, &d.aCz, c) ||
+      dalloc(b, &d.ruq, c) || dalloc(b, &d.aNbr, c * AMCD_NB) || dalloc(b, &d.aCold, c) ||
+      dalloc(b, &d.aScan, c) || dalloc(b, &d.gAct, c) || dalloc(b, &d.gScan, c) || dalloc(b, &d.gExt, c) ||
+      dalloc(b, &d.gVhat, c) || dalloc(b, &d.gN, c) || dalloc(b, &d.gDead, c) || dalloc(b, &d.gGrow, c) ||
+      dalloc(b, &d.cD, c * AMCD_NB) || dalloc(b, &d.cF, c * AMCD_NB) || dalloc(b, &d.cR, c * AMCD_NB) ||
+      dalloc(b, &d.cTgt, c * AMCD_NB) || dalloc(b, &d.cK, c * AMCD_NB))
+    return -1;
+  return 0;
+}
+
+static int push_inst(amcd_batch* b) {
+  HIPT(hipMemcpyAsync(b->d_inst, b->h.data(), sizeof(Inst) * b->n, hipMemcpyHostToDevice, b->stream));
+  return 0;
+}
+
+extern "C" {
+
+const char* amcd_last_error() { return g_dev_err.c_str(); }
+
+int amcd_device_name(char* buf, int len) {
+  hipDeviceProp_t pr;
+  int dev = 0;
+  HIPT(hipGetDevice(&dev));
+  HIPT(hipGetDeviceProperties(&pr, dev));
+  snprintf(buf, len, "%s (%s, %d CUs)", pr.name, pr.gcnArchName, pr.multiProcessorCount);
+  return 0;
+}
+
+int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int errlen) {
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    snprintf(err, errlen, "no HIP device");
+    return -1;
+  }
+  amcd_batch* b = new amcd_batch();
+  b->n = n;
+  b->h.resize(n);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    snprintf(err, errlen, "hipStreamCreate failed");
+    delete b;
+    return -1;
+  }
+  for (int q = 0; q < 16; ++q) hipEventCreate(&b->ev[q]);
+  b->bstartG.assign(n + 1, 0);
+  b->bstartZ.assign(n + 1, 0);
+  b->astart.assign(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    Inst& d = b->h[i];
+    memset(&d, 0, sizeof(Inst));
+    d.p = p[i];
+    long long N = (long long)p[i].nx * p[i].ny * p[i].nz;
+    if (N <= 0 || N > 0x7fffffffLL - 4096) {
+      snprintf(err, errlen, "domain %d: voxel count %lld out of range", i, N);
+      amcd_destroy(b);
+      return -1;
+    }
+    d.N = (int)N;
+    d.nxy = p[i].nx * p[i].ny;
+    d.nblkZ = (d.N + ZB - 1) / ZB;
+    d.nblkG = (d.N + GB - 1) / GB;
+    size_t Nn = (size_t)d.N;
+    if (dalloc(b, &d.vs, Nn) || dalloc(b, &d.gid, Nn) || dalloc(b, &d.ext, Nn) || dalloc(b, &d.cx, Nn) ||
+        dalloc(b, &d.cy, Nn) || dalloc(b, &d.cz, Nn) || dalloc(b, &d.T, Nn) || dalloc(b, &d.bj, Nn) ||
+        dalloc(b, &d.amap, Nn) || dalloc(b, &d.bcA, (size_t)d.nblkZ) || dalloc(b, &d.bcM, (size_t)d.nblkZ) ||
+        dalloc(b, &d.ls, 1)) {
+      snprintf(err, errlen, "%s", g_dev_err.c_str());
+      amcd_destroy(b);
+      return -1;
+    }
+    hipMemsetAsync(d.vs, 0, Nn, b->stream);
+    hipMemsetAsync(d.gid, 0, Nn * 4, b->stream);
+    hipMemsetAsync(d.ext, 0, Nn * 8, b->stream);
+    hipMemsetAsync(d.cx, 0, Nn * 8, b->stream);
+    hipMemsetAsync(d.cy, 0, Nn * 8, b->stream);
+    hipMemsetAsync(d.cz, 0, Nn * 8, b->stream);
+    hipMemsetAsync(d.T, 0, Nn * 8, b->stream);
+    hipMemsetAsync(d.bj, 0, Nn, b->stream);
+    hipMemsetAsync(d.amap, 0xff, Nn * 4, b->stream);
+    hipMemsetAsync(d.ls, 0, sizeof(LoopState), b->stream);
+    int acap = (int)std::min<long long>(N, std::max<long long>(1 << 16, N / 8));
+    if (alloc_active(b, d, acap)) {
+      snprintf(err, errlen, "%s", g_dev_err.c_str());
+      amcd_destroy(b);
+      return -1;
+    }
+    d.gcap = 0;
+    d.rot = nullptr;
+    d.poolcap = 0;
+    d.pool = nullptr;
+    b->bstartG[i + 1] = b->bstartG[i] + d.nblkG;
+    b->bstartZ[i + 1] = b->bstartZ[i] + d.nblkZ;
+  }
+  if (dalloc(b, &b->d_inst, (size_t)n) || dalloc(b, &b->d_bstartG, (size_t)n + 1) ||
+      dalloc(b, &b->d_bstartZ, (size_t)n + 1) || dalloc(b, &b->d_astart, (size_t)n + 1) ||
+      dalloc(b, &b->d_steps, (size_t)n) || dalloc(b, &b->d_outs, (size_t)n)) {
+    snprintf(err, errlen, "%s", g_dev_err.c_str());
+    amcd_destroy(b);
+    return -1;
+  }
+  if (hipHostMalloc((void**)&b->h_steps, sizeof(amcd_step) * n, 0) != hipSuccess ||
+      hipHostMalloc((void**)&b->h_outs, sizeof(amcd_out) * n, 0) != hipSuccess) {
+    snprintf(err, errlen, "hipHostMalloc failed");
+    amcd_destroy(b);
+    return -1;
+  }
+  hipMemcpyAsync(b->d_bstartG, b->bstartG.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
+  hipMemcpyAsync(b->d_bstartZ, b->bstartZ.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
+  if (push_inst(b) || hipStreamSynchronize(b->stream) != hipSuccess) {
+    snprintf(err, errlen, "%s", g_dev_err.c_str());
+    amcd_destroy(b);
+    return -1;
+  }
+  *out = b;
+  return 0;
+}
+
+void amcd_destroy(amcd_batch* b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (void* q : b->allocs) hipFree(q);
+  for (int q = 0; q < 16; ++q)
+    if (b->ev[q]) hipEventDestroy(b->ev[q]);
+  if (b->h_steps) hipHostFree(b->h_steps);
+  if (b->h_outs) hipHostFree(b->h_outs);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+void amcd_set_timing(amcd_batch* b, int on) { b->timing = on != 0; }
+
+int amcd_upload(amcd_batch* b, int i, const int* gid, const int* vstate, const double* ext, const double* cen,
+                const int* bj) {
+  if (i < 0 || i >= b->n) return -1;
+  Inst& d = b->h[i];
+  HIPT(hipStreamSynchronize(b->stream));
+  size_t N = d.N;
+  std::vector<uint8_t> tmp(N);
+  if (vstate) {
+    for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(vstate[v] & 3);
+    HIPT(hipMemcpy(d.vs, tmp.data(), N, hipMemcpyHostToDevice));
+  }
+  if (bj) {
+    for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(bj[v] ? 1 : 0);
+    HIPT(hipMemcpy(d.bj, tmp.data(), N, hipMemcpyHostToDevice));
+  }
+  if (gid) HIPT(hipMemcpy(d.gid, gid, N * 4, hipMemcpyHostToDevice));
+  if (ext) HIPT(hipMemcpy(d.ext, ext, N * 8, hipMemcpyHostToDevice));
+  if (cen) {
+    std::vector<double> c(N);
+    double* dst[3] = {d.cx, d.cy, d.cz};
+    for (int q = 0; q < 3; ++q) {
+      for (size_t v = 0; v < N; ++v) c[v] = cen[3 * v + q];
+      HIPT(hipMemcpy(dst[q], c.data(), N * 8, hipMemcpyHostToDevice));
+    }
+  }
+  return 0;
+}
+
+int amcd_download(amcd_batch* b, int i, int* gid, int* vstate, double* ext, double* cen, int* bj, double* T) {
+  if (i < 0 || i >= b->n) return -1;
+  Inst& d = b->h[i];
+  HIPT(hipStreamSynchronize(b->stream));
+  size_t N = d.N;
+  std::vector<uint8_t> tmp(N);
+  if (vstate) {
+    HIPT(hipMemcpy(tmp.data(), d.vs, N, hipMemcpyDeviceToHost));
+    for (size_t v = 0; v < N; ++v) vstate[v] = tmp[v] & 3;
+  }
+  if (bj) {
+    HIPT(hipMemcpy(tmp.data(), d.bj, N, hipMemcpyDeviceToHost));
+    for (size_t v = 0; v < N; ++v) bj[v] = tmp[v];
+  }
+  if (gid) HIPT(hipMemcpy(gid, d.gid, N * 4, hipMemcpyDeviceToHost));
+  if (ext) HIPT(hipMemcpy(ext, d.ext, N * 8, hipMemcpyDeviceToHost));
+  if (T) HIPT(hipMemcpy(T, d.T, N * 8, hipMemcpyDeviceToHost));
+  if (cen) {
+    std::vector<double> c(N);
+    double* src[3] = {d.cx, d.cy, d.cz};
+    for (int q = 0; q < 3; ++q) {
+      HIPT(hipMemcpy(c.data(), src[q], N * 8, hipMemcpyDeviceToHost));
+      for (size_t v = 0; v < N; ++v) cen[3 * v + q] = c[v];
+    }
+  }
+  return 0;
+}
+
+static int ensure_grains(amcd_batch* b, Inst& d, int need) {
+  if (need <= d.gcap) return 0;
+  int ncap = std::max(need, std::max(1024, d.gcap * 2));
+  double* nr = nullptr;
+  HIPT(hipMalloc(&nr, sizeof(double) * 9 * (size_t)ncap));
+  if (d.rot) {
+    HIPT(hipMemcpyAsync(nr, d.rot, sizeof(double) * 9 * (size_t)d.gcap, hipMemcpyDeviceToDevice, b->stream));
+    HIPT(hipStreamSynchronize(b->stream));
+    for (auto& q : b->allocs)
+      if (q == d.rot) q = nr;
+    hipFree(d.rot);
+  } else {
+    b->allocs.push_back(nr);
+  }
+  d.rot = nr;
+  d.gcap = ncap;
+  return push_inst(b);
+}
+
+int amcd_set_rot(amcd_batch* b, int i, const double* rot9, int ngrain) {
+  Inst& d = b->h[i];
+  if (ensure_grains(b, d, ngrain + 256)) return -1;
+  if (ngrain > 0)
+    HIPT(hipMemcpyAsync(d.rot, rot9, sizeof(double) * 9 * (size_t)ngrain, hipMemcpyHostToDevice, b->stream));
+  HIPT(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int amcd_get_rot(amcd_batch* b, int i, double* rot9, int ngrain) {
+  Inst& d = b->h[i];
+  if (ngrain > d.gcap) return -1;
+  HIPT(hipStreamSynchronize(b->stream));
+  HIPT(hipMemcpy(rot9, d.rot, sizeof(double) * 9 * (size_t)ngrain, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n) {
+  Inst& d = b->h[i];
+  if (n > d.poolcap) {
+    double* np = nullptr;
+    int ncap = std::max(n, 64);
+    HIPT(hipMalloc(&np, sizeof(double) * 9 * (size_t)ncap));
+    b->allocs.push_back(np);
+    d.pool = np;
+    d.poolcap = ncap;
+    if (push_inst(b)) return -1;
+  }
+  if (n > 0) HIPT(hipMemcpyAsync(d.pool, rot9, sizeof(double) * 9 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+  return 0;
+}
+
+// grain-table capacity for the coming step: current grains + pool size
+int amcd_reserve_grains(amcd_batch* b, int i, int need) { return ensure_grains(b, b->h[i], need); }
+
+static int put_steps(amcd_batch* b, const amcd_step* steps) {
+  memcpy(b->h_steps, steps, sizeof(amcd_step) * b->n);
+  HIPT(hipMemcpyAsync(b->d_steps, b->h_steps, sizeof(amcd_step) * b->n, hipMemcpyHostToDevice, b->stream));
+  return 0;
+}
+
+static void tmark(amcd_batch* b, int q) {
+  if (b->timing) hipEventRecord(b->ev[q], b->stream);
+}
+
+static int run_temperature(amcd_batch* b, int setliquid) {
+  int nb = b->bstartG[b->n];
+  hipLaunchKernelGGL(k_temperature, dim3(nb), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, b->n, b->d_steps,
+                     b->d_outs, setliquid);
+  HIPT(hipGetLastError());
+  return 0;
+}
+
+int amcd_temperature(amcd_batch* b, const amcd_step* steps) {
+  if (put_steps(b, steps)) return -1;
+  HIPT(hipMemsetAsync(b->d_outs, 0, sizeof(amcd_out) * b->n, b->stream));
+  return run_temperature(b, 0);
+}
+
+__global__ void k_loop_rng(Inst* I, const amcd_step* S) {
+  const Inst& d = I[blockIdx.x];
+  LoopState* ls = d.ls;
+  if (threadIdx.x != 0 || ls->done) return;
+  unsigned int x0 = lcg_seed(30ull * (unsigned long long)S[blockIdx.x].tInd + d.p.seed1);
+  ls->rng = mod_m((unsigned long long)x0 * lcg_pow(2ull * (unsigned long long)ls->Na));
+  ls->draws = 2ll * ls->Na;
+}
+
+static int launch_update(amcd_batch* b, bool fused_temperature) {
+  const int n = b->n;
+  tmark(b, 0);
+  if (fused_temperature) {
+    if (run_temperature(b, 1)) return -1;
+  } else {
+    hipLaunchKernelGGL(k_setliquid, dim3(b->bstartG[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, n,
+                       b->d_steps);
+  }
+  tmark(b, 1);
+  hipLaunchKernelGGL(k_solid_to_mushy, dim3(b->bstartG[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, n,
+                     b->d_steps);
+  tmark(b, 2);
+  hipLaunchKernelGGL(k_active_flags, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
+                     b->d_steps);
+  hipLaunchKernelGGL(k_scan, dim3(n), dim3(1024), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
+  hipLaunchKernelGGL(k_compact, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
+                     b->d_steps);
+  hipLaunchKernelGGL(k_loop_rng, dim3(n), dim3(64), 0, b->stream, b->d_inst, b->d_steps);
+  tmark(b, 3);
+  HIPT(hipGetLastError());
+  return 0;
+}
+
+static int read_outs(amcd_batch* b, amcd_out* outs) {
+  HIPT(hipMemcpyAsync(b->h_outs, b->d_outs, sizeof(amcd_out) * b->n, hipMemcpyDeviceToHost, b->stream));
+  HIPT(hipStreamSynchronize(b->stream));
+  memcpy(outs, b->h_outs, sizeof(amcd_out) * b->n);
+  return 0;
+}
+
+// After the scan the active counts are known; the active-cell kernels are sized by
+// the capacity (idle threads exit), so no host round trip is needed mid-step.
+static int launch_active(amcd_batch* b) {
+  const int n = b->n;
+  // active-cell block list sized by capacity
+  b->astart[0] = 0;
+  for (int i = 0; i < n; ++i) b->astart[i + 1] = b->astart[i] + (b->h[i].acap + GB - 1) / GB;
+  HIPT(hipMemcpyAsync(b->d_astart, b->astart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
+                     b->d_outs);
+  tmark(b, 4);
+  hipLaunchKernelGGL(k_event_loop, dim3(n), dim3(EL_NT), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
+  tmark(b, 5);
+  HIPT(hipGetLastError());
+  return 0;
+}
+
+static int launch_finish(amcd_batch* b) {
+  const int n = b->n;
+  hipLaunchKernelGGL(k_scatter, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n);
+  hipLaunchKernelGGL(k_mushy_to_solid, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n);
+  tmark(b, 6);
+  HIPT(hipGetLastError());
+  return 0;
+}
+
+static void collect_timing(amcd_batch* b) {
+  if (!b->timing) return;
+  hipEventSynchronize(b->ev[6]);
+  for (int q = 0; q < 6; ++q) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]);
+    b->last_ms[q] = ms;
+  }
+}
+
+int amcd_read_outs(amcd_batch* b, amcd_out* outs) { return read_outs(b, outs); }
+
+int amcd_last_timings(amcd_batch* b, double* ms, int n) {
+  for (int q = 0; q < n && q < 6; ++q) ms[q] = b->last_ms[q];
+  return 0;
+}
+
+// growth of the active capacity after an overflow; the step is then repeated (the
+// stages before the compaction are idempotent for the same temperature field)
+static int grow_active(amcd_batch* b, const amcd_out* outs) {
+  bool any = false;
+  for (int i = 0; i < b->n; ++i) {
+    if (!outs[i].overflow) continue;
+    Inst& d = b->h[i];
+    int need = outs[i].n_active;
+    int ncap = (int)std::min<long long>(d.N, std::max<long long>((long long)need * 3 / 2, (long long)d.acap * 2));
+    // old arrays stay allocated until destroy; they are simply no longer referenced
+    if (alloc_active(b, d, ncap)) return -1;
+    any = true;
+  }
+  if (any && push_inst(b)) return -1;
+  return 0;
+}
+
+static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, bool fused) {
+  if (put_steps(b, steps)) return -1;
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    if (fused || attempt > 0) {
+      // fresh output records; the temperature kernel sets any_melt
+      HIPT(hipMemsetAsync(b->d_outs, 0, sizeof(amcd_out) * b->n, b->stream));
+    }
+    if (launch_update(b, fused || attempt > 0)) return -1;
+    if (launch_active(b)) return -1;
+    if (read_outs(b, outs)) return -1;
+    bool ovf = false;
+    for (int i = 0; i < b->n; ++i) ovf = ovf || outs[i].overflow;
+    if (!ovf) break;
+    if (grow_active(b, outs)) return -1;
+  }
+  bool pend = false;
+  for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
+  if (!pend) {
+    if (launch_finish(b)) return -1;
+    HIPT(hipStreamSynchronize(b->stream));
+    collect_timing(b);
+  }
+  return 0;
+}
+
+int amcd_update_voxels(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
+  return update_common(b, steps, outs, false);
+}
+
+int amcd_step_fused(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
+  return update_common(b, steps, outs, true);
+}
+
+int amcd_resume(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
+  if (put_steps(b, steps)) return -1;
+  hipLaunchKernelGGL(k_event_loop, dim3(b->n), dim3(EL_NT), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
+  HIPT(hipGetLastError());
+  if (read_outs(b, outs)) return -1;
+  bool pend = false;
+  for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
+  if (!pend) {
+    if (launch_finish(b)) return -1;
+    HIPT(hipStreamSynchronize(b->stream));
+  }
+  return 0;
+}
+
+int amcd_grain_presence(amcd_batch* b, int i, int ngrain, int* present) {
+  Inst& d = b->h[i];
+  int* dp = nullptr;
+  HIPT(hipMalloc(&dp, sizeof(int) * (size_t)(ngrain + 1)));
+  HIPT(hipMemsetAsync(dp, 0, sizeof(int) * (size_t)(ngrain + 1), b->stream));
+  hipLaunchKernelGGL(k_grain_presence, dim3((d.N + 255) / 256), dim3(256), 0, b->stream, d.gid, d.N, dp, ngrain);
+  HIPT(hipMemcpyAsync(present, dp, sizeof(int) * (size_t)ngrain, hipMemcpyDeviceToHost, b->stream));
+  HIPT(hipStreamSynchronize(b->stream));
+  HIPT(hipFree(dp));
+  return 0;
+}
+
+int amcd_remap_grains(amcd_batch* b, int i, const int* remap, int ngrain) {
+  Inst& d = b->h[i];
+  int* dr = nullptr;
+  HIPT(hipMalloc(&dr, sizeof(int) * (size_t)(ngrain + 1)));
+  HIPT(hipMemcpyAsync(dr, remap, sizeof(int) * (size_t)ngrain, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_remap, dim3((d.N + 255) / 256), dim3(256), 0, b->stream, d.gid, d.N, dr, ngrain);
+  HIPT(hipStreamSynchronize(b->stream));
+  HIPT(hipFree(dr));
+  return 0;
+}
+
+int amcd_write_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, const int* gtmp) {
+  Inst& d = b->h[i];
+  int nl = ilaserLoc - iz1;
+  size_t cnt = (size_t)d.nxy * (size_t)(nl > 0 ? nl : 0);
+  int* dg = nullptr;
+  HIPT(hipMalloc(&dg, sizeof(int) * (cnt + 1)));
+  if (cnt) HIPT(hipMemcpyAsync(dg, gtmp, sizeof(int) * cnt, hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_write_layer, dim3((d.N + 255) / 256), dim3(256), 0, b->stream, b->d_inst, i, iz1, ilaserLoc,
+                     gbase, dg);
+  HIPT(hipStreamSynchronize(b->stream));
+  HIPT(hipFree(dg));
+  return 0;
+}
+
+// the capture shape factor evaluated by the host build of the device math
+int amcd_host_shape_factor(const double* dnx, int n, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = amc_shape_factor(dnx[3 * i], dnx[3 * i + 1], dnx[3 * i + 2]);
+  return 0;
+}
+
+// engine stream of libstdc++'s uniform_real_distribution<double>(0,1) over
+// minstd_rand0(seed), as the device computes it (for known-answer tests)
+int amcd_uniform_stream(unsigned long long seed, int n, double* out) {
+  unsigned int x = lcg_seed(seed);
+  for (int q = 0; q < n; ++q) {
+    unsigned int x1 = lcg_next(x), x2 = lcg_next(x1);
+    x = x2;
+    out[q] = canonical(x1, x2);
+  }
+  return 0;
+}
+// the same stream from jump-ahead positions (the ruq path of k_compact)
+int amcd_uniform_jump(unsigned long long seed, int n, double* out) {
+  unsigned int x0 = lcg_seed(seed);
+  for (int a = 0; a < n; ++a) {
+    unsigned int x1 = mod_m((unsigned long long)x0 * lcg_pow(2ull * a + 1));
+    out[a] = canonical(x1, lcg_next(x1));
+  }
+  return 0;
+}
+
+}  // extern "C"

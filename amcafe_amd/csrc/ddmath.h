@@ -1,0 +1,140 @@
+// ddmath.h -- correctly rounded FP64 atan2 / sin / cos for the capture geometry.
+//
+// The CA capture test (S/VoxelsCA.C:415-424, 548-551, 591) evaluates
+//     th = atan2(|dy|,|dx|)  folded to [0, pi/4]
+//     ph = atan2(sqrt(dx^2+dy^2), |dz|)  folded to [pi/4, pi/2]
+//     f  = sqrt(cos(th) * sin(ph))
+// Each function here computes its result in double-double arithmetic (relative
+// error below 2^-100) and returns the nearest double, i.e. the correctly rounded
+// value except when the exact result lies within 2^-100 of a rounding midpoint.
+// sqrt and division are IEEE operations on the device and need no help.
+//
+// Arguments are restricted to what the capture geometry produces:
+//   amc_atan2_q1(y, x):  y >= 0, x >= 0, finite
+//   amc_sin_q1 / amc_cos_q1(x):  0 <= x <= pi/2
+// Error-free transformations use explicit fma(); the file must be compiled with
+// -ffp-contract=off so the compiler does not fuse anything else.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "ddtables.h"
+
+struct amc_dd {
+  double hi, lo;
+};
+
+__host__ __device__ static inline amc_dd amc_two_sum(double a, double b) {
+  double s = a + b;
+  double bb = s - a;
+  double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__host__ __device__ static inline amc_dd amc_fast_two_sum(double a, double b) {  // |a| >= |b|
+  double s = a + b;
+  double e = b - (s - a);
+  return {s, e};
+}
+__host__ __device__ static inline amc_dd amc_two_prod(double a, double b) {
+  double p = a * b;
+  double e = fma(a, b, -p);
+  return {p, e};
+}
+__host__ __device__ static inline amc_dd amc_dd_add(amc_dd x, amc_dd y) {
+  amc_dd s = amc_two_sum(x.hi, y.hi);
+  amc_dd t = amc_two_sum(x.lo, y.lo);
+  s.lo += t.hi;
+  s = amc_fast_two_sum(s.hi, s.lo);
+  s.lo += t.lo;
+  return amc_fast_two_sum(s.hi, s.lo);
+}
+__host__ __device__ static inline amc_dd amc_dd_neg(amc_dd x) { return {-x.hi, -x.lo}; }
+__host__ __device__ static inline amc_dd amc_dd_mul(amc_dd x, amc_dd y) {
+  amc_dd p = amc_two_prod(x.hi, y.hi);
+  p.lo += x.hi * y.lo + x.lo * y.hi;
+  return amc_fast_two_sum(p.hi, p.lo);
+}
+__host__ __device__ static inline amc_dd amc_dd_mul_d(amc_dd x, double y) {
+  amc_dd p = amc_two_prod(x.hi, y);
+  p.lo += x.lo * y;
+  return amc_fast_two_sum(p.hi, p.lo);
+}
+__host__ __device__ static inline amc_dd amc_dd_div(amc_dd x, amc_dd y) {
+  double q1 = x.hi / y.hi;
+  amc_dd r = amc_dd_add(x, amc_dd_neg(amc_dd_mul_d(y, q1)));
+  double q2 = r.hi / y.hi;
+  r = amc_dd_add(r, amc_dd_neg(amc_dd_mul_d(y, q2)));
+  double q3 = r.hi / y.hi;
+  amc_dd q = amc_fast_two_sum(q1, q2);
+  return amc_dd_add(q, amc_dd{q3, 0.0});
+}
+__host__ __device__ static inline amc_dd amc_tab(const double (*t)[2], int i) { return {t[i][0], t[i][1]}; }
+
+// Horner evaluation of sum_k c[k] * z^k in double-double.
+__host__ __device__ static inline amc_dd amc_poly(const double (*c)[2], int n, amc_dd z) {
+  amc_dd p = amc_tab(c, n - 1);
+  for (int k = n - 2; k >= 0; --k) p = amc_dd_add(amc_dd_mul(p, z), amc_tab(c, k));
+  return p;
+}
+
+// atan(q) for a double-double q in [0, 1]: table point c = i/64 plus the series of
+// atan(r), r = (q - c) / (1 + q c), |r| <= 1/128 (terms through r^23).
+__host__ __device__ static inline amc_dd amc_atan_dd01(amc_dd q) {
+  int i = (int)rint(q.hi * AMC_DD_TAB);
+  double c = (double)i / AMC_DD_TAB;
+  amc_dd num = amc_dd_add(q, amc_dd{-c, 0.0});
+  amc_dd den = amc_dd_add(amc_dd{1.0, 0.0}, amc_dd_mul_d(q, c));
+  amc_dd r = amc_dd_div(num, den);
+  amc_dd r2 = amc_dd_mul(r, r);
+  amc_dd s = amc_dd_mul(r, amc_poly(kAtanCoef, 12, r2));
+  return amc_dd_add(amc_tab(kAtanTab, i), s);
+}
+
+// atan2(y, x) rounded to nearest, for y >= 0, x >= 0 (both finite).
+__host__ __device__ static inline double amc_atan2_q1(double y, double x) {
+  if (y == 0.0) return 0.0;
+  if (x == 0.0) return kPi2[0][0];
+  bool swap = y > x;
+  double num = swap ? x : y, den = swap ? y : x;
+  amc_dd q = amc_dd_div(amc_dd{num, 0.0}, amc_dd{den, 0.0});
+  amc_dd a = amc_atan_dd01(q);
+  if (swap) a = amc_dd_add(amc_tab(kPi2, 0), amc_dd_neg(a));
+  return a.hi + a.lo;
+}
+
+// sin(x), cos(x) rounded to nearest for 0 <= x <= pi/2: x = k/64 + r, |r| <= 1/128,
+// Taylor series of sin r, cos r in double-double, then the addition formulas.
+__host__ __device__ static inline void amc_sincos_dd(double x, amc_dd* s, amc_dd* c) {
+  int k = (int)rint(x * AMC_DD_TAB);
+  double r = x - (double)k / AMC_DD_TAB;  // exact
+  amc_dd rr = {r, 0.0};
+  amc_dd r2 = amc_two_prod(r, r);
+  amc_dd sr = amc_dd_mul(rr, amc_poly(kSinCoef, 10, r2));
+  amc_dd cr = amc_poly(kCosCoef, 10, r2);
+  amc_dd sk = amc_tab(kSinTab, k), ck = amc_tab(kCosTab, k);
+  if (s) *s = amc_dd_add(amc_dd_mul(sk, cr), amc_dd_mul(ck, sr));
+  if (c) *c = amc_dd_add(amc_dd_mul(ck, cr), amc_dd_neg(amc_dd_mul(sk, sr)));
+}
+__host__ __device__ static inline double amc_sin_q1(double x) {
+  amc_dd s;
+  amc_sincos_dd(x, &s, nullptr);
+  return s.hi + s.lo;
+}
+__host__ __device__ static inline double amc_cos_q1(double x) {
+  amc_dd c;
+  amc_sincos_dd(x, nullptr, &c);
+  return c.hi + c.lo;
+}
+
+// Shape factor of the decentred-octahedron capture test, S/VoxelsCA.C:415-418, 424:
+//   pow(cos(th) * sin(ph), .5) with th, ph from the centre-to-centroid vector.
+// The operation order (products, sum, folds) follows the reference exactly; the
+// libm calls are the correctly rounded versions above.
+__host__ __device__ static inline double amc_shape_factor(double dx, double dy, double dz) {
+  const double pi4 = 0x1.921fb54442d18p-1, pi2 = 0x1.921fb54442d18p+0;  // M_PI/4.0, M_PI/2.0
+  double th = amc_atan2_q1(fabs(dy), fabs(dx));
+  th > pi4 ? th = pi2 - th : th;
+  double rho = sqrt(dx * dx + dy * dy);
+  double ph = amc_atan2_q1(rho, fabs(dz));
+  ph < pi4 ? ph = pi2 - ph : ph;
+  return sqrt(amc_cos_q1(th) * amc_sin_q1(ph));
+}

@@ -1,0 +1,132 @@
+/* amcafe.h -- C ABI of the MI355X-native CAFE time step.
+ *
+ * Two levels, both plain C (pointers, sizes, int status codes; no C++ types):
+ *
+ *  1. Seam level: a device-resident CA domain that replaces the per-step member
+ *     functions the reference's driver calls,
+ *       TempField::AnalyticTempCurr   src/src-mpi/TempField.h:22, TempField.C:139-217
+ *       VoxelsCA::UpdateVoxels        src/src-mpi/VoxelsCA.h:35,  VoxelsCA.C:232-655
+ *     and, with the whole domain in one GPU's HBM, the halo exchange they rely on,
+ *       Partition::PassInformation    src/src-mpi/Partition.h:13-14, Partition.C:414-449
+ *     State is uploaded / downloaded in the reference's layouts: int32 gID and
+ *     vState, double extents, double centroidOct (xyz interleaved), int32 BuildjID,
+ *     all indexed by the linear voxel id x + nX*y + nX*nY*z (1 MPI rank).
+ *
+ *  2. Driver level: the reference's main loop (src/src-mpi/main1.C:22-98) for one
+ *     or many input decks at once -- deck parsing (Grid.C:165-318), base plate
+ *     (BasePlate.C:41-97), powder layers (VoxelsCA.C:887-982), CleanLayer
+ *     (VoxelsCA.C:800-839), SetBuild (VoxelsCA.C:76-115) and the HDF5 output
+ *     layout of WriteToHDF1 (VoxelsCA.C:1592-1856).  Several decks advance as one
+ *     batch: one launch per stage for all of them.
+ *
+ * Every function returns AMCAFE_OK (0) or a negative code; amcafe_last_error()
+ * gives a message for the calling thread.  No C++ exception crosses this ABI.
+ * A handle must be used by one host thread at a time.
+ */
+#ifndef AMCAFE_H
+#define AMCAFE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMCAFE_OK 0
+#define AMCAFE_ERR_ARG (-1)     /* bad argument or call order */
+#define AMCAFE_ERR_DEVICE (-2)  /* HIP runtime failure or no device */
+#define AMCAFE_ERR_DECK (-3)    /* deck could not be parsed / is incomplete */
+#define AMCAFE_ERR_STATE (-4)   /* state the path cannot handle (see message) */
+#define AMCAFE_ERR_IO (-5)      /* output file could not be written */
+
+const char* amcafe_last_error(void);
+/* device description, e.g. "AMD Instinct MI355X (gfx950, 256 CUs)" */
+int amcafe_device_name(char* buf, int len);
+
+/* ---------------------------------------------------------------------------
+ * 1. Seam level
+ */
+typedef struct amcafe_domain amcafe_domain;
+
+/* Grid / TempField / VoxelsCA members the time step reads (Grid.h:42-70). */
+typedef struct {
+  int nX[3];
+  double dX;           /* isotropic spacing, deck keyword dx (Grid.C:200-206) */
+  double tL, tS, T0;   /* liquidus, solidus, ambient (T0 = 300 in the reference) */
+  double meltparam[4]; /* deck keyword meltparam a b c d (TempField.C:107-114) */
+  double bmV, bmDelT;  /* beam speed, time step */
+  double rNmax;        /* deck keyword r */
+  double Avel, nvel;   /* deck keyword dendritegrowth2 */
+} amcafe_domain_desc;
+
+/* Laser state for one temperature evaluation (TempField.C:146-171, Grid.h:42-70). */
+typedef struct {
+  double x, y, z;      /* lcoor[2*ispvec[isp]]-offset[0], ...+1]-offset[1], ilaserLoc*dX-offset[2] */
+  double dirp;         /* scan direction as in TempField.C:167-171 */
+  double gth;          /* grid rotation angle (Grid::gth) */
+  int ilaserLoc;       /* Grid::ilaserLoc */
+} amcafe_laser;
+
+/* Per-step counters of UpdateVoxels. */
+typedef struct {
+  int n_active;        /* interface cells (the reference's Na) */
+  int n_mushy0;        /* mushy cells at the start of the capture loop */
+  int n_events;        /* capture-loop iterations that captured or nucleated */
+  int n_captures, n_nucleations;
+  int n_growing;       /* cells that could capture at some point of the step */
+  int nGrain;          /* grains after the step */
+  long long draws;     /* minstd_rand0 outputs consumed */
+  double tinc;         /* time advanced by the capture loop */
+} amcafe_stats;
+
+int amcafe_domain_create(const amcafe_domain_desc* desc, amcafe_domain** out);
+void amcafe_domain_destroy(amcafe_domain* d);
+/* any pointer may be NULL (that field is left unchanged) */
+int amcafe_domain_upload(amcafe_domain* d, const int* gID, const int* vState, const double* extents,
+                         const double* centroidOct, const int* BuildjID);
+int amcafe_domain_download(amcafe_domain* d, int* gID, int* vState, double* extents, double* centroidOct,
+                           int* BuildjID, double* TempCurr);
+/* grain orientations cTheta (omega, axis) per grain, as VoxelsCA::cTheta */
+int amcafe_domain_set_grains(amcafe_domain* d, const double* cTheta, int nGrain);
+int amcafe_domain_get_grains(amcafe_domain* d, double* cTheta, int* nGrain);
+/* TempField::AnalyticTempCurr: TempCurr and BuildjID on the device; *any_melt
+ * is the end-of-scan input "some voxel at or above tL" (TempField.C:202-205). */
+int amcafe_analytic_temp_curr(amcafe_domain* d, const amcafe_laser* laser, int* any_melt);
+/* VoxelsCA::UpdateVoxels on the current TempCurr; tInd = Grid::tInd. */
+int amcafe_update_voxels(amcafe_domain* d, int tInd, int ilaserLoc, amcafe_stats* stats);
+
+/* ---------------------------------------------------------------------------
+ * 2. Driver level
+ */
+typedef struct amcafe_sim amcafe_sim;
+
+typedef struct {
+  int nX[3];
+  int nGrain;
+  int tInd;            /* Grid::tInd (completed steps) */
+  int isp, indlayer, ilaserLoc, nlayerTot;
+  int done;            /* the run's layer loop has finished (main1.C:75) */
+  double time;
+} amcafe_info;
+
+/* n_decks deck texts (the .in file contents); they advance together. */
+int amcafe_sim_create(const char* const* deck_texts, int n_decks, amcafe_sim** out);
+void amcafe_sim_destroy(amcafe_sim* s);
+/* one iteration of main1.C:62-93 for every unfinished deck; *n_running gets the
+ * number of decks still running afterwards */
+int amcafe_sim_step(amcafe_sim* s, int* n_running);
+int amcafe_sim_info(amcafe_sim* s, int inst, amcafe_info* info);
+int amcafe_sim_stats(amcafe_sim* s, int inst, amcafe_stats* stats);
+/* reference layouts; NULL pointers are skipped; Build is VoxelsCA::Build */
+int amcafe_sim_get_fields(amcafe_sim* s, int inst, int* gID, int* vState, double* TempCurr, double* extents,
+                          double* centroidOct, int* BuildjID, int* Build);
+int amcafe_sim_get_grains(amcafe_sim* s, int inst, double* cTheta, int* nGrain);
+/* WriteToHDF1 layout: Step0/{dims,VoxelDX,gID,vState,Build,Temperature,IPFz,IPFx,IPFy,angleAxis} */
+int amcafe_sim_write_hdf5(amcafe_sim* s, int inst, const char* path);
+/* per-stage device time of the last step in ms when enabled: temperature, solid->mushy,
+ * active list, growing-cell setup, capture loop, scatter + mushy->solid */
+int amcafe_sim_set_timing(amcafe_sim* s, int on);
+int amcafe_sim_timings(amcafe_sim* s, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

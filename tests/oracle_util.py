@@ -1,0 +1,139 @@
+"""ctypes access to the test-only oracle (oracle/_build/liboracle.so)."""
+import ctypes
+import os
+import shutil
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+_P, _I = ctypes.c_void_p, ctypes.c_int
+
+
+def _load(path):
+    L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
+    L.orc_create.restype = _P
+    L.orc_create.argtypes = [ctypes.c_char_p]
+    L.orc_last_error.restype = ctypes.c_char_p
+    for f in ("orc_destroy", "orc_temperature", "orc_update_voxels", "orc_state_in", "orc_state_out"):
+        getattr(L, f).argtypes = [_P]
+    for f in ("orc_step", "orc_finish_step", "orc_done", "orc_ngrain"):
+        getattr(L, f).argtypes = [_P]
+        getattr(L, f).restype = _I
+    L.orc_dims.argtypes = [_P, _P]
+    L.orc_get_ctheta.argtypes = [_P, _P]
+    L.orc_set_ctheta.argtypes = [_P, _P, _I]
+    L.orc_get_fields.argtypes = [_P] + [_P] * 7
+    L.orc_set_fields.argtypes = [_P] + [_P] * 6
+    L.orc_get_scalars.argtypes = [_P, _P]
+    L.orc_set_laser.argtypes = [_P, _I, _I, _I]
+    L.orc_stats.argtypes = [_P, _P, _P]
+    L.orc_generate_samples.argtypes = [_I, ctypes.c_uint, _P]
+    L.orc_uniform_stream.argtypes = [ctypes.c_ulong, _I, _P]
+    L.orc_rot_mat.argtypes = [ctypes.c_double, _P, _P]
+    L.orc_shape_factor_n.argtypes = [_P, _I, _P]
+    L.orc_set_crmath.argtypes = [_I]
+    return L
+
+
+_copies = {}
+
+
+def oracle_lib(crmath):
+    """A private copy of the oracle per math mode (the mode is a library global)."""
+    key = bool(crmath)
+    if key not in _copies:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built; run __graft_entry__.build()")
+        d = tempfile.mkdtemp(prefix="amcafe_orc_")
+        p = os.path.join(d, f"liboracle_{int(key)}.so")
+        shutil.copy(ORACLE_SO, p)
+        L = _load(p)
+        L.orc_set_crmath(1 if key else 0)
+        _copies[key] = L
+    return _copies[key]
+
+
+class Oracle:
+    """The reference's driver loop restated on the CPU (glibc or correctly rounded math)."""
+
+    def __init__(self, deck_text, crmath=False):
+        self.L = oracle_lib(crmath)
+        self.h = self.L.orc_create(deck_text.encode())
+        if not self.h:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        d = (ctypes.c_int * 3)()
+        self.L.orc_dims(self.h, d)
+        self.nX = list(d)
+        self.N = d[0] * d[1] * d[2]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_destroy(self.h)
+            self.h = None
+
+    def step(self):
+        return self.L.orc_step(self.h)
+
+    def temperature(self):
+        self.L.orc_temperature(self.h)
+
+    def update_voxels(self):
+        self.L.orc_update_voxels(self.h)
+
+    def finish_step(self):
+        return self.L.orc_finish_step(self.h)
+
+    def fields(self):
+        N = self.N
+        o = dict(gID=np.zeros(N, np.int32), vState=np.zeros(N, np.int32), TempCurr=np.zeros(N),
+                 extents=np.zeros(N), centroidOct=np.zeros(3 * N), BuildjID=np.zeros(N, np.int32),
+                 Build=np.zeros(N, np.int32))
+        self.L.orc_get_fields(self.h, *[o[k].ctypes.data for k in
+                                         ("gID", "vState", "TempCurr", "extents", "centroidOct", "BuildjID", "Build")])
+        return o
+
+    def set_fields(self, gID=None, vState=None, TempCurr=None, extents=None, centroidOct=None, BuildjID=None):
+        c = lambda a, t: None if a is None else np.ascontiguousarray(a, dtype=t)
+        arrs = [c(gID, np.int32), c(vState, np.int32), c(TempCurr, np.float64), c(extents, np.float64),
+                c(centroidOct, np.float64), c(BuildjID, np.int32)]
+        self.L.orc_set_fields(self.h, *[None if a is None else a.ctypes.data for a in arrs])
+        self._keep = arrs
+
+    def grains(self):
+        n = self.L.orc_ngrain(self.h)
+        ct = np.zeros(4 * n)
+        self.L.orc_get_ctheta(self.h, ct.ctypes.data)
+        return ct
+
+    def set_grains(self, ct):
+        ct = np.ascontiguousarray(ct, dtype=np.float64)
+        self.L.orc_set_ctheta(self.h, ct.ctypes.data, len(ct) // 4)
+
+    def scalars(self):
+        o = np.zeros(14)
+        self.L.orc_get_scalars(self.h, o.ctypes.data)
+        keys = ("time", "tInd", "isp", "ilaserLoc", "indlayer", "inewscanflg", "inewlayerflg", "gth", "nlayerTot",
+                "Nzh", "NpT", "Nsd", "Ntd", "TtInd")
+        return dict(zip(keys, o.tolist()))
+
+    def stats(self):
+        o = np.zeros(6, np.int64)
+        t = np.zeros(1)
+        self.L.orc_stats(self.h, o.ctypes.data, t.ctypes.data)
+        return dict(n_active=int(o[0]), n_mushy0=int(o[1]), n_events=int(o[2]), n_captures=int(o[3]),
+                    n_nucleations=int(o[4]), draws=int(o[5]), tinc=float(t[0]))
+
+    def done(self):
+        return self.L.orc_done(self.h)
+
+
+def deck_path(name):
+    return os.path.join(ROOT, "decks", name)
+
+
+def read(name):
+    with open(deck_path(name)) as f:
+        return f.read()
