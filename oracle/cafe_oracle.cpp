@@ -40,6 +40,7 @@
 #include <random>
 #include <sstream>
 #include <string>
+#include <unordered_map>
 #include <vector>
 #include <quadmath.h>
 
@@ -55,6 +56,40 @@ static double m_sin(double x) { return g_crmath ? (double)sinq((__float128)x) : 
 // pow(x,.5) is a libm call in the reference (GCC does not fold it to sqrt without
 // -funsafe-math-optimizations); sqrt is its correctly rounded value.
 static double m_powh(double x) { return g_crmath ? std::sqrt(x) : std::pow(x, .5); }
+
+// pow(cos th * sin ph, .5) of S/VoxelsCA.C:415-418, 424 (also :548-551, 591), a pure
+// function of the centre-to-centroid vector; memoised because the reference's scan
+// re-evaluates the same vectors in every iteration of the capture loop.
+struct Key3 {
+  uint64_t a, b, c;
+  bool operator==(const Key3& o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct Key3Hash {
+  size_t operator()(const Key3& k) const {
+    uint64_t h = k.a * 0x9E3779B97F4A7C15ull ^ (k.b + 0x632BE59BD9B4E019ull + (k.a << 6)) ^ (k.c * 0xC2B2AE3D27D4EB4Full);
+    return (size_t)(h ^ (h >> 29));
+  }
+};
+static std::unordered_map<Key3, double, Key3Hash> g_shape_memo;
+static double shape_factor_raw(const double* dnx) {
+  double th = m_atan2(std::fabs(dnx[1]), std::fabs(dnx[0]));
+  th > M_PI / 4.0 ? th = M_PI / 2.0 - th : th;
+  double ph = m_atan2(m_powh(dnx[0] * dnx[0] + dnx[1] * dnx[1]), std::fabs(dnx[2]));
+  ph < M_PI / 4.0 ? ph = M_PI / 2.0 - ph : ph;
+  return m_powh(m_cos(th) * m_sin(ph));
+}
+static double shape_factor(const double* dnx) {
+  Key3 k;
+  std::memcpy(&k.a, &dnx[0], 8);
+  std::memcpy(&k.b, &dnx[1], 8);
+  std::memcpy(&k.c, &dnx[2], 8);
+  auto it = g_shape_memo.find(k);
+  if (it != g_shape_memo.end()) return it->second;
+  if (g_shape_memo.size() > (8u << 20)) g_shape_memo.clear();
+  double f = shape_factor_raw(dnx);
+  g_shape_memo.emplace(k, f);
+  return f;
+}
 
 typedef std::minstd_rand0 Engine;  // std::default_random_engine in libstdc++ (random.h:1604)
 
@@ -665,14 +700,10 @@ struct Sim {
           dnx[0] = (double(jx0) + .5) * g.dX[0] - Cen[3 * (size_t)i1];
           dnx[1] = (double(jx1) + .5) * g.dX[1] - Cen[3 * (size_t)i1 + 1];
           dnx[2] = (double(jx2) + .5) * g.dX[2] - Cen[3 * (size_t)i1 + 2];
-          double th = m_atan2(std::fabs(dnx[1]), std::fabs(dnx[0]));
-          th > M_PI / 4.0 ? th = M_PI / 2.0 - th : th;
-          double ph = m_atan2(m_powh(dnx[0] * dnx[0] + dnx[1] * dnx[1]), std::fabs(dnx[2]));
-          ph < M_PI / 4.0 ? ph = M_PI / 2.0 - ph : ph;
           double d0 = std::fabs(R[0][0] * dnx[0] + R[1][0] * dnx[1] + R[2][0] * dnx[2]);
           double d1 = std::fabs(R[0][1] * dnx[0] + R[1][1] * dnx[1] + R[2][1] * dnx[2]);
           double d2 = std::fabs(R[0][2] * dnx[0] + R[1][2] * dnx[1] + R[2][2] * dnx[2]);
-          double dr = m_powh(m_cos(th) * m_sin(ph)) * (d0 + d1 + d2) - ExtA[i1];
+          double dr = shape_factor(dnx) * (d0 + d1 + d2) - ExtA[i1];
           double timeUntil = dr / vhatvec[j];
           double ruqs = .5 * (ruq[i1] + ruq[i4]);
           if (timeUntil <= DtMin) {
@@ -735,10 +766,7 @@ struct Sim {
         locX[0] = R[0][0] * dnx[0] + R[1][0] * dnx[1] + R[2][0] * dnx[2];
         locX[1] = R[0][1] * dnx[0] + R[1][1] * dnx[1] + R[2][1] * dnx[2];
         locX[2] = R[0][2] * dnx[0] + R[1][2] * dnx[1] + R[2][2] * dnx[2];
-        double th = m_atan2(std::fabs(dnx[1]), std::fabs(dnx[0]));
-        th > M_PI / 4.0 ? th = M_PI / 2.0 - th : th;
-        double ph = m_atan2(m_powh(dnx[0] * dnx[0] + dnx[1] * dnx[1]), std::fabs(dnx[2]));
-        ph < M_PI / 4.0 ? ph = M_PI / 2.0 - ph : ph;
+        const double fshape = shape_factor(dnx);
         double dr = std::fabs(locX[0]) + std::fabs(locX[1]) + std::fabs(locX[2]);
         double sd[6][3];
         for (int q = 0; q < 6; ++q)
@@ -769,7 +797,7 @@ struct Sim {
         double L13 = .5 * (std::min(d1J, sq3 * l) + std::min(dJ3, sq3 * l));
         double Lmud = sq3 * (s23 * std::max(L12, L13));
         double xiL = 1.0;
-        ExtA[i1] = m_powh(m_cos(th) * m_sin(ph)) * Lmud * xiL;
+        ExtA[i1] = fshape * Lmud * xiL;
         const double* s0 = S0[sInd[jInd][k0]];
         dnx[0] = v0[0] - Lmud * s0[0];
         dnx[1] = v0[1] - Lmud * s0[1];
@@ -912,7 +940,10 @@ extern "C" {
 
 static std::string g_err;
 const char* orc_last_error() { return g_err.c_str(); }
-void orc_set_crmath(int on) { orc::g_crmath = on; }
+void orc_set_crmath(int on) {
+  orc::g_crmath = on;
+  orc::g_shape_memo.clear();
+}
 
 void* orc_create(const char* deck_text) {
   orc::Sim* s = new orc::Sim();
@@ -1011,13 +1042,7 @@ void orc_rot_mat(double omega, const double* ax, double* R9) {
   for (int i = 0; i < 9; ++i) R9[i] = R[i / 3][i % 3];
 }
 void orc_shape_factor_n(const double* dnx, int n, double* out);
-double orc_shape_factor(const double* dnx) {  // pow(cos th * sin ph, .5), S/VoxelsCA.C:415-424
-  double th = orc::m_atan2(std::fabs(dnx[1]), std::fabs(dnx[0]));
-  th > M_PI / 4.0 ? th = M_PI / 2.0 - th : th;
-  double ph = orc::m_atan2(orc::m_powh(dnx[0] * dnx[0] + dnx[1] * dnx[1]), std::fabs(dnx[2]));
-  ph < M_PI / 4.0 ? ph = M_PI / 2.0 - ph : ph;
-  return orc::m_powh(orc::m_cos(th) * orc::m_sin(ph));
-}
+double orc_shape_factor(const double* dnx) { return orc::shape_factor_raw(dnx); }
 void orc_shape_factor_n(const double* dnx, int n, double* out) {
   for (int i = 0; i < n; ++i) out[i] = orc_shape_factor(dnx + 3 * i);
 }
