@@ -19,13 +19,15 @@ class AmcafeError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [("n_active", ctypes.c_int), ("n_mushy0", ctypes.c_int), ("n_events", ctypes.c_int),
                 ("n_captures", ctypes.c_int), ("n_nucleations", ctypes.c_int), ("n_growing", ctypes.c_int),
-                ("nGrain", ctypes.c_int), ("draws", ctypes.c_longlong), ("tinc", ctypes.c_double)]
+                ("nGrain", ctypes.c_int), ("draws", ctypes.c_longlong), ("tinc", ctypes.c_double),
+                ("slot_visits", ctypes.c_longlong), ("cand_evals", ctypes.c_longlong)]
 
 
 class Info(ctypes.Structure):
     _fields_ = [("nX", ctypes.c_int * 3), ("nGrain", ctypes.c_int), ("tInd", ctypes.c_int), ("isp", ctypes.c_int),
                 ("indlayer", ctypes.c_int), ("ilaserLoc", ctypes.c_int), ("nlayerTot", ctypes.c_int),
-                ("done", ctypes.c_int), ("time", ctypes.c_double)]
+                ("done", ctypes.c_int), ("inewscanflg", ctypes.c_int), ("inewlayerflg", ctypes.c_int),
+                ("time", ctypes.c_double), ("gth", ctypes.c_double)]
 
 
 class DomainDesc(ctypes.Structure):

@@ -61,6 +61,8 @@ typedef struct {
   double tinc;
   int overflow;        // active list exceeded capacity
   int err_nonanalytic; // a growing cell had a temperature with no host-computed velocity
+  long long slot_visits;  // growing cells with live candidates, summed over the loop's scans
+  long long cand_evals;   // capture times evaluated, summed over the loop's scans
 } amcd_out;
 
 typedef struct amcd_batch amcd_batch;

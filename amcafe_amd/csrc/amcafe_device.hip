@@ -644,7 +644,7 @@ __global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* 
   unsigned int rng = 0;
   double tinc = 0;
   int vSind2 = 0, nGrain = 0, pool_used = 0, events = 0, captures = 0, nucs = 0;
-  long long draws = 0;
+  long long draws = 0, visits = 0, evals = 0;  // per-thread work counters
   if (tid == 0) {
     rng = ls->rng;
     tinc = ls->tinc;
@@ -678,6 +678,7 @@ __global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* 
       const int ns = s_nslots;
       for (int s = tid; s < ns; s += EL_NT) {
         if (d.gDead[s]) continue;
+        visits++;
         const double E = d.gExt[s], V = d.gVhat[s];
         const int n = d.gN[s];
         const size_t cb = (size_t)s * AMCD_NB;
@@ -687,6 +688,7 @@ __global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* 
           int tg = d.cTgt[cb + q];
           if (d.aVS[tg] != 1) continue;
           live = true;
+          evals++;
           double dr = d.cD[cb + q] - E;
           double t = dr / V;
           if (!(t <= 1e6)) continue;
@@ -836,6 +838,8 @@ __global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* 
   const bool suspended = (s_go == -1);
   if (!suspended)
     for (int s = tid; s < ns; s += EL_NT) d.aExt[d.gAct[s]] = d.gExt[s];
+  atomicAdd((unsigned long long*)&O[blockIdx.x].slot_visits, (unsigned long long)visits);
+  atomicAdd((unsigned long long*)&O[blockIdx.x].cand_evals, (unsigned long long)evals);
   if (tid == 0) {
     ls->rng = rng;
     ls->tinc = tinc;

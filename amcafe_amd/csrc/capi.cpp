@@ -131,6 +131,8 @@ static void to_stats(const amcd_out& o, amcafe_stats* s) {
   s->nGrain = o.nGrain;
   s->draws = o.draws;
   s->tinc = o.tinc;
+  s->slot_visits = o.slot_visits;
+  s->cand_evals = o.cand_evals;
 }
 
 // ====================================================================================
@@ -513,7 +515,10 @@ extern "C" int amcafe_sim_info(amcafe_sim* s, int i, amcafe_info* info) {
   info->ilaserLoc = r.g.ilaserLoc;
   info->nlayerTot = r.nlayerTot;
   info->done = r.done ? 1 : 0;
+  info->inewscanflg = r.g.inewscanflg;
+  info->inewlayerflg = r.g.inewlayerflg;
   info->time = r.g.time;
+  info->gth = r.g.gth;
   return AMCAFE_OK;
 }
 

@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""bench.py -- CA cell-updates/s of the CAFE time step on one MI355X.
+
+A step is one iteration of the reference's main loop (src/src-mpi/main1.C:62-93):
+temperature, UpdateVoxels, laser / layer bookkeeping.  One cell-update is one voxel
+advanced one step, whatever its state, so value = nX*nY*nZ * steps / wall time of the
+timed steps, with the domain resident in HBM before the timed region starts.
+
+Default workload: c1_raster (512x512x128 voxels, 4 layers of 30 um, 3-track
+bidirectional raster, 90 deg layer rotation), the largest single-GPU configuration.
+
+Also reported, in the same JSON line:
+  roofline      the dominant kernel's algorithmic bytes per launch / its mean launch
+                duration (HIP events on the library's stream), against 8 TB/s HBM3E
+  cpu_baseline  the oracle (a one-rank CPU port of the reference path) timed on the
+                host for a few steps starting from the GPU's state before the timed region
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CA cell-updates/s on 1 MI355X + %HBM peak; batched vs single; adjoint cost"
+HBM_PEAK_GBS = 8000.0
+STAGES = ("temperature+SetLiquid3", "solid->mushy", "active list", "growing setup", "capture loop",
+          "scatter+mushy->solid")
+KERNEL_NAMES = ("k_temperature", "k_solid_to_mushy", "k_active_flags/k_scan/k_compact", "k_active_setup",
+                "k_event_loop", "k_scatter/k_mushy_to_solid")
+WORKLOADS = {
+    "c1_raster": "512x512x128 voxels, dx 2.5 um, 4 layers of 30 um, 3-track bidirectional raster, "
+                 "hatch 60 um, 90 deg rotation, 250 W, 1.0 m/s, FP64",
+    "c0_single_track": "128x64x32 voxels, dx 2.5 um, one layer, one track at 0.8 m/s, FP64",
+}
+
+
+def algorithmic_bytes(stage, nvox, st):
+    """Bytes one launch of a stage must move (each array element read or written once)."""
+    if stage == 0:   # T write 8 + vState r/w 2 + BuildjID r/w 2 (device layout)
+        return 12.0 * nvox
+    if stage == 1:   # vState read + conditional write
+        return 2.0 * nvox
+    if stage == 4:   # capture loop: per live growing cell per scan ExtA r/w 16, vhat 8, scan
+        # key + state 12; per live candidate D 8, ruqs 8, target 4 + its state 1
+        return 36.0 * st["slot_visits"] + 21.0 * st["cand_evals"]
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--workload", default="c1_raster", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched form shards "
+              "instances over GPUs with no communication, so N GPUs is N independent runs.", file=sys.stderr)
+        sys.exit(2)
+
+    import amcafe_amd
+    deck = amcafe_amd.read_deck(os.path.join(ROOT, "decks", args.workload + ".in"))
+    t_setup = time.perf_counter()
+    sim = amcafe_amd.Sim(deck)
+    t_setup = time.perf_counter() - t_setup
+    info = sim.info()
+    nvox = info["nX"][0] * info["nX"][1] * info["nX"][2]
+    for _ in range(args.warmup):
+        sim.step()
+    snap = None
+    if not args.no_cpu:
+        snap = (sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID")), sim.grains(0),
+                sim.info(0))
+    sim.set_timing(True)
+    stage_ms = np.zeros(6)
+    tot = dict(slot_visits=0, cand_evals=0, n_events=0, n_active=0, n_growing=0)
+    steps = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        running = sim.step()
+        steps += 1
+        stage_ms += sim.timings()
+        st = sim.stats()
+        for k in tot:
+            tot[k] += st[k]
+        if running == 0:
+            break
+    wall = time.perf_counter() - t0
+    value = nvox * steps / wall
+
+    dom = int(np.argmax(stage_ms))
+    launch_ms = stage_ms[dom] / steps
+    abytes = algorithmic_bytes(dom, nvox, tot)
+    roof = None
+    if abytes is not None:
+        per_launch = abytes / steps
+        achieved = per_launch / (launch_ms * 1e-3) / 1e9
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{args.workload}.json")
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "kernel": KERNEL_NAMES[dom], "launch_ms": round(launch_ms, 4),
+                "algorithmic_bytes_per_launch": round(per_launch, 1)}
+
+    cpu = None
+    if snap is not None:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_util import Oracle
+        orc = Oracle(deck, crmath=False, bare=True)
+        f, g, inf = snap
+        orc.load_state(f, g, inf)
+        tc = time.perf_counter()
+        for _ in range(args.cpu_steps):
+            orc.temperature()
+            orc.update_voxels()
+            orc.finish_step()
+        tc = time.perf_counter() - tc
+        cpu = {"value": round(nvox * args.cpu_steps / tc, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of {args.workload} from the state after {args.warmup} warmup "
+                         f"steps, one thread, {tc:.1f} s"}
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * wall / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "description": WORKLOADS[args.workload], "voxels": nvox,
+                   "instances": 1, "setup_s": round(t_setup, 2),
+                   "stage_ms_per_step": dict(zip(STAGES, np.round(stage_ms / steps, 4).tolist())),
+                   "events_per_step": round(tot["n_events"] / steps, 1),
+                   "active_cells_per_step": round(tot["n_active"] / steps, 1),
+                   "growing_cells_per_step": round(tot["n_growing"] / steps, 1)},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

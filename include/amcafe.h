@@ -75,6 +75,8 @@ typedef struct {
   int nGrain;          /* grains after the step */
   long long draws;     /* minstd_rand0 outputs consumed */
   double tinc;         /* time advanced by the capture loop */
+  long long slot_visits; /* growing cells with live candidates, summed over the loop's scans */
+  long long cand_evals;  /* capture times evaluated, summed over the loop's scans */
 } amcafe_stats;
 
 int amcafe_domain_create(const amcafe_domain_desc* desc, amcafe_domain** out);
@@ -104,7 +106,9 @@ typedef struct {
   int tInd;            /* Grid::tInd (completed steps) */
   int isp, indlayer, ilaserLoc, nlayerTot;
   int done;            /* the run's layer loop has finished (main1.C:75) */
+  int inewscanflg, inewlayerflg;
   double time;
+  double gth;          /* current grid rotation (Grid::gth) */
 } amcafe_info;
 
 /* n_decks deck texts (the .in file contents); they advance together. */

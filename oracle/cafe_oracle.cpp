@@ -406,7 +406,7 @@ struct Sim {
     }
   }
 
-  int setup(const std::string& deck, std::string& err) {
+  int setup(const std::string& deck, std::string& err, bool bare = false) {
     if (g.parse(deck, err)) return -1;
     g.init();
     N = g.nX[0] * g.nX[1] * g.nX[2];
@@ -435,6 +435,7 @@ struct Sim {
     gID.assign(N, 0); vState.assign(N, 0); Build.assign(N, 0);
     extents.assign(N, 0.0); centroidOct.assign(3 * (size_t)N, 0.0);
     genlayer.seed(seed1);
+    if (bare) return 0;  // state, grains and laser are set by the caller
     initialize_voxels();
     // driver (S/main1.C:59-61)
     nlayerTot = int(ceil((double)(g.nX[2] - Nzh) / (double)g.nZlayer));
@@ -950,6 +951,11 @@ void* orc_create(const char* deck_text) {
   if (s->setup(deck_text, g_err)) { delete s; return nullptr; }
   return s;
 }
+void* orc_create_bare(const char* deck_text) {
+  orc::Sim* s = new orc::Sim();
+  if (s->setup(deck_text, g_err, true)) { delete s; return nullptr; }
+  return s;
+}
 void orc_destroy(void* h) { delete (orc::Sim*)h; }
 void orc_temperature(void* h) { ((orc::Sim*)h)->analytic_temp(); }
 void orc_update_voxels(void* h) { ((orc::Sim*)h)->update_voxels(); }
@@ -1015,6 +1021,11 @@ void orc_set_laser(void* h, int isp, int ilaserLoc, int tInd) {
   orc::Sim* s = (orc::Sim*)h;
   s->g.isp = isp; s->g.ilaserLoc = ilaserLoc; s->g.tInd = tInd;
   s->g.time = tInd * s->DelT;
+}
+void orc_set_grid(void* h, double time, double gth, int indlayer, int inewscanflg, int inewlayerflg) {
+  orc::Sim* s = (orc::Sim*)h;
+  s->g.time = time; s->g.gth = gth; s->g.indlayer = indlayer;
+  s->g.inewscanflg = inewscanflg; s->g.inewlayerflg = inewlayerflg;
 }
 // stats of the last UpdateVoxels: Na, mushy at loop start, iterations, captures, nucleations, draws
 void orc_stats(void* h, long* o, double* tinc) {

@@ -16,6 +16,9 @@ def _load(path):
     L = ctypes.CDLL(path, mode=ctypes.RTLD_LOCAL)
     L.orc_create.restype = _P
     L.orc_create.argtypes = [ctypes.c_char_p]
+    L.orc_create_bare.restype = _P
+    L.orc_create_bare.argtypes = [ctypes.c_char_p]
+    L.orc_set_grid.argtypes = [_P, ctypes.c_double, ctypes.c_double, _I, _I, _I]
     L.orc_last_error.restype = ctypes.c_char_p
     for f in ("orc_destroy", "orc_temperature", "orc_update_voxels", "orc_state_in", "orc_state_out"):
         getattr(L, f).argtypes = [_P]
@@ -59,9 +62,9 @@ def oracle_lib(crmath):
 class Oracle:
     """The reference's driver loop restated on the CPU (glibc or correctly rounded math)."""
 
-    def __init__(self, deck_text, crmath=False):
+    def __init__(self, deck_text, crmath=False, bare=False):
         self.L = oracle_lib(crmath)
-        self.h = self.L.orc_create(deck_text.encode())
+        self.h = (self.L.orc_create_bare if bare else self.L.orc_create)(deck_text.encode())
         if not self.h:
             raise RuntimeError(self.L.orc_last_error().decode())
         d = (ctypes.c_int * 3)()
@@ -128,6 +131,17 @@ class Oracle:
 
     def done(self):
         return self.L.orc_done(self.h)
+
+    def load_state(self, fields, grains, info):
+        """Take over a state produced elsewhere (fields in the reference layouts, the grain
+        table and the laser / layer scalars of the driver)."""
+        self.set_fields(gID=fields["gID"], vState=fields["vState"], TempCurr=fields.get("TempCurr"),
+                        extents=fields["extents"], centroidOct=fields["centroidOct"],
+                        BuildjID=fields["BuildjID"])
+        self.set_grains(grains)
+        self.L.orc_set_laser(self.h, info["isp"], info["ilaserLoc"], info["tInd"])
+        self.L.orc_set_grid(self.h, info["time"], info["gth"], info["indlayer"], info["inewscanflg"],
+                            info["inewlayerflg"])
 
 
 def deck_path(name):
