@@ -34,8 +34,6 @@
 
 #define ZB 1024     // voxels per compaction block (one per thread)
 #define GB 256      // threads per grid-wide block
-#define EL_NT 256   // threads of the event-loop workgroup
-#define EL_NW (EL_NT / 64)
 
 // -------------------------------------------------------------------------------------
 // device structures
@@ -417,7 +415,7 @@ __device__ static inline void centre(const Inst& d, int v, double* c) {
 
 // Capture candidate of growing cell a (centroid c, rotation R) towards liquid voxel
 // vt: D = pow(cos th sin ph,.5) * |R^T dnx|_1 and the factor f (S/VoxelsCA.C:409-424).
-__device__ static inline void candidate(const Inst& d, const double* c, const double* R, int vt, double* D,
+__device__ __noinline__ static void candidate(const Inst& d, const double* c, const double* R, int vt, double* D,
                                         double* f) {
   double ct[3];
   centre(d, vt, ct);
@@ -442,6 +440,8 @@ __device__ static inline double growth_velocity(const amcd_params& p, double T, 
 // neighbour, all active neighbours below tL; S/VoxelsCA.C:396-397), its slot in the
 // growing list and its capture candidates.  Slot order is irrelevant: the event loop
 // compares candidates by (time, ruqs, scan position, neighbour rank).
+__device__ static inline bool cand_less(double Da, double Ra, int ka, double Db, double Rb, int kb);
+
 __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart, int ninst, amcd_out* O) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
   const Inst& d = I[ii];
@@ -516,6 +516,27 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
     d.cK[cb + n] = (uint8_t)q;
     ++n;
   }
+  // sort the list by (D, ruqs, key): the capture loop reads it front to back
+  const int kb = d.aScan[a] * 32;
+  for (int x = 1; x < n; ++x) {
+    double D = d.cD[cb + x], f = d.cF[cb + x], r = d.cR[cb + x];
+    int t = d.cTgt[cb + x];
+    uint8_t kk = d.cK[cb + x];
+    int y = x - 1;
+    while (y >= 0 && cand_less(D, r, kb + kk, d.cD[cb + y], d.cR[cb + y], kb + d.cK[cb + y])) {
+      d.cD[cb + y + 1] = d.cD[cb + y];
+      d.cF[cb + y + 1] = d.cF[cb + y];
+      d.cR[cb + y + 1] = d.cR[cb + y];
+      d.cTgt[cb + y + 1] = d.cTgt[cb + y];
+      d.cK[cb + y + 1] = d.cK[cb + y];
+      --y;
+    }
+    d.cD[cb + y + 1] = D;
+    d.cF[cb + y + 1] = f;
+    d.cR[cb + y + 1] = r;
+    d.cTgt[cb + y + 1] = t;
+    d.cK[cb + y + 1] = kk;
+  }
   d.gN[slot] = (uint8_t)n;
 }
 
@@ -539,7 +560,7 @@ __device__ static inline void project_point_line(const double* A, const double* 
 __device__ static inline double dist3(const double* p, const double* q) {
   return sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]));
 }
-__device__ static void recentre(const Inst& d, int vcap, int vnew, const double* cen, const double* R, double f,
+__device__ __noinline__ static void recentre(const Inst& d, int vcap, int vnew, const double* cen, const double* R, double f,
                                 double* ncen, double* next) {
   const amcd_params& p = d.p;
   int jx0, jx1, jx2, jy0, jy1, jy2;
@@ -606,7 +627,7 @@ __device__ static void recentre(const Inst& d, int vcap, int vnew, const double*
 
 struct Best {
   double t, r;
-  long long key;
+  int key;
   int cand;
 };
 __device__ static inline bool better(const Best& a, const Best& b) {  // a strictly before b
@@ -624,27 +645,210 @@ __device__ static inline Best shfl_best(const Best& b, int lane_mask) {
   o.cand = __shfl_xor(b.cand, lane_mask);
   return o;
 }
+// candidate order inside a growing cell's list: (D, ruqs, key), NaN distances last
+__device__ static inline bool cand_less(double Da, double Ra, int ka, double Db, double Rb, int kb) {
+  bool na = Da != Da, nb = Db != Db;
+  if (na != nb) return nb;
+  if (!na && Da != Db) return Da < Db;
+  if (Ra != Rb) return Ra < Rb;
+  return ka < kb;
+}
 
-// The capture / nucleation loop of one instance (S/VoxelsCA.C:381-610).
-// Each iteration: every live candidate's time t = (D - ExtA)/vhat, the lexicographic
-// minimum of (t, ruqs, scan order) over all of them, one uniform draw, then either a
-// nucleation at the captured voxel or growth of every growing cell by vhat*Dt and a
-// capture with octahedron re-centring.  The workgroup keeps going while tinc < DelT.
-__global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* S, amcd_out* O) {
-  __shared__ Best sb[EL_NW];
-  __shared__ int s_go, s_nuc, s_tgt, s_slot, s_cand, s_nslots, s_newslot;
-  __shared__ double s_dt;
+// First live entry at or after position `from` of slot s's sorted list, and the
+// distance of the next live entry (+inf if none).  Entries before the first live one
+// stay dead: a captured voxel never becomes liquid again within the step.
+struct SlotHead {
+  double D1, R1, D2;
+  int key1, c1, tgt1;
+  bool alive;
+};
+__device__ static inline bool live_target(const Inst& d, int t, int skip) { return t != skip && d.aVS[t] == 1; }
+__device__ __noinline__ static void slot_head(const Inst& d, int s, int from, int skip, SlotHead& h) {
+  const int n = d.gN[s];
+  const size_t cb = (size_t)s * AMCD_NB;
+  int p = from;
+  while (p < n && !live_target(d, d.cTgt[cb + p], skip)) ++p;
+  if (p >= n) {
+    h.alive = false;
+    h.D1 = h.D2 = __builtin_inf();
+    h.R1 = 0;
+    h.key1 = 0;
+    h.c1 = (int)cb + n;
+    h.tgt1 = -1;
+    return;
+  }
+  h.alive = true;
+  h.c1 = (int)(cb + p);
+  h.tgt1 = d.cTgt[cb + p];
+  h.D1 = d.cD[cb + p];
+  h.R1 = d.cR[cb + p];
+  h.key1 = d.gScan[s] * 32 + d.cK[cb + p];
+  int q = p + 1;
+  while (q < n && !live_target(d, d.cTgt[cb + q], skip)) ++q;
+  h.D2 = (q < n) ? d.cD[cb + q] : __builtin_inf();
+}
+
+// Exact best of a slot when its first two live entries tie in time: walk the block of
+// live entries with the same time (times are non-decreasing along the sorted list).
+__device__ __noinline__ static Best slot_walk(const Inst& d, int s, int c1, double E, double V, Best b, long long& evals) {
+  const int n = d.gN[s];
+  const size_t cb = (size_t)s * AMCD_NB;
+  const int kb = d.gScan[s] * 32;
+  for (int p = c1 - (int)cb + 1; p < n; ++p) {
+    if (d.aVS[d.cTgt[cb + p]] != 1) continue;
+    evals++;
+    double t = (d.cD[cb + p] - E) / V;
+    if (!(t == b.t)) break;
+    Best c;
+    c.t = t;
+    c.r = d.cR[cb + p];
+    c.key = kb + d.cK[cb + p];
+    c.cand = (int)(cb + p);
+    if (better(c, b)) b = c;
+  }
+  return b;
+}
+
+// The capture / nucleation loop of one instance (S/VoxelsCA.C:381-610), one workgroup
+// of NT threads per instance, split in two roles that meet at three barriers per
+// iteration:
+//   wave 0      the engine: picks the winner of the argmin, draws the uniform,
+//               nucleates or captures with octahedron re-centring, and builds the
+//               sorted candidate list of the newly mushy voxel;
+//   waves 1..   the owners: thread 64+j owns growing cells j, j+(NT-64), ... (K of
+//               them) and keeps each cell's half-diagonal E, velocity V and the head
+//               of its sorted candidate list in registers.
+// Per iteration the owners compute every cell's capture time t = (D1 - E)/V of its
+// first live candidate (the minimum: t is non-decreasing along a list sorted by D;
+// exact ties in t are resolved by walking the list), reduce (t, ruqs, scan order),
+// then, while the engine works on the captured voxel, grow every cell by V*Dt and
+// re-read the lists whose first live target was captured.
+struct EngineShared {
+  Best sb[16];
+  int stop, nuc, i1, cand, nslots, next, newslot;
+  double dt;
+  double mE, mV, mD1, mR1, mD2;
+  int mkey1, mc1, mtgt1;
+};
+
+template <int NT, int K>
+__device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, long long& visits, long long& evals) {
+  constexpr int NO = NT - 64;  // owner threads
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int me = tid - 64;
+  double E[K], V[K], D1[K], R1[K], D2[K];
+  int key1[K], c1[K], tgt1[K];
+  bool alive[K];
+  const bool loaded = go != -2;
+  const int ns0 = sh.nslots;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int s = me + k * NO;
+    alive[k] = false;
+    E[k] = V[k] = D1[k] = R1[k] = D2[k] = 0;
+    key1[k] = c1[k] = tgt1[k] = 0;
+    if (s < ns0 && loaded) {
+      E[k] = d.gExt[s];
+      V[k] = d.gVhat[s];
+      SlotHead h;
+      slot_head(d, s, 0, -1, h);
+      D1[k] = h.D1; R1[k] = h.R1; D2[k] = h.D2; key1[k] = h.key1; c1[k] = h.c1; tgt1[k] = h.tgt1;
+      alive[k] = h.alive;
+    }
+  }
+  while (go > 0) {
+    if (go == 1) {
+      Best b;
+      b.t = __builtin_inf();
+      b.r = __builtin_inf();
+      b.key = 0x7fffffff;
+      b.cand = -1;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (!alive[k]) continue;
+        visits++;
+        evals++;
+        const double t1 = (D1[k] - E[k]) / V[k];
+        if (!(t1 <= 1e6)) continue;
+        Best c;
+        c.t = t1;
+        c.r = R1[k];
+        c.key = key1[k];
+        c.cand = c1[k];
+        const double t2 = (D2[k] - E[k]) / V[k];
+        if (t2 == t1) c = slot_walk(d, me + k * NO, c1[k], E[k], V[k], c, evals);
+        if (better(c, b)) b = c;
+      }
+      for (int m = 32; m >= 1; m >>= 1) {
+        Best o = shfl_best(b, m);
+        if (better(o, b)) b = o;
+      }
+      if (lane == 0) sh.sb[w] = b;
+      __syncthreads();  // 1: candidates in
+      __syncthreads();  // 2: decision out
+      if (sh.stop) break;
+      const int i1 = sh.i1;
+      if (!sh.nuc) {  // growth of every cell that was growing at this scan (S/VoxelsCA.C:511-516)
+        const double Dt = sh.dt;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!alive[k]) continue;
+          double e = E[k] + V[k] * Dt;
+          E[k] = (e < 0.0) ? 0.0 : e;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {  // lists whose first live target is being captured now
+        if (alive[k] && tgt1[k] == i1) {
+          const int s = me + k * NO;
+          SlotHead h;
+          slot_head(d, s, c1[k] - s * AMCD_NB + 1, i1, h);
+          D1[k] = h.D1; R1[k] = h.R1; D2[k] = h.D2; key1[k] = h.key1; c1[k] = h.c1; tgt1[k] = h.tgt1;
+          alive[k] = h.alive;
+        }
+      }
+    }
+    __syncthreads();  // 3: capture done, new cell in the mailbox
+    if (sh.newslot == 2) {
+      const int slot = sh.nslots - 1;
+      if (slot % NO == me) {
+        const int kk = slot / NO;
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          if (k == kk) {
+            E[k] = sh.mE; V[k] = sh.mV; D1[k] = sh.mD1; R1[k] = sh.mR1; D2[k] = sh.mD2;
+            key1[k] = sh.mkey1; c1[k] = sh.mc1; tgt1[k] = sh.mtgt1; alive[k] = true;
+          }
+      }
+    }
+    go = sh.next;
+  }
+  if (loaded) {
+    const int ns = sh.nslots;  // stable: the engine has stopped
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int s = me + k * NO;
+      if (s < ns) d.gExt[s] = E[k];
+    }
+  }
+}
+
+template <int NT, int K>
+__global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, amcd_out* O) {
+  constexpr int NW = NT / 64;
+  constexpr int CAP = (NT - 64) * K;
+  __shared__ EngineShared sh;
+  __shared__ int s_go;
   const Inst& d = I[blockIdx.x];
   LoopState* ls = d.ls;
   if (ls->done) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const amcd_params& p = d.p;
   const amcd_step& st = S[blockIdx.x];
-  // thread 0 owns the scalar loop state
   unsigned int rng = 0;
   double tinc = 0;
   int vSind2 = 0, nGrain = 0, pool_used = 0, events = 0, captures = 0, nucs = 0;
-  long long draws = 0, visits = 0, evals = 0;  // per-thread work counters
+  long long draws = 0, visits = 0, evals = 0;
   if (tid == 0) {
     rng = ls->rng;
     tinc = ls->tinc;
@@ -655,189 +859,181 @@ __global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* 
     captures = ls->captures;
     nucs = ls->nucs;
     draws = ls->draws;
-    s_nslots = ls->nslots;
+    sh.nslots = ls->nslots;
     s_go = 1;
-    if (ls->pending) {  // resume: apply the nucleation decided before the pool ran out
-      s_nuc = 1;
-      s_tgt = ls->pend_tgt;
-      s_dt = 0.0;
+    if (ls->nslots > CAP) {
+      s_go = -2;  // more growing cells than this tier holds
+    } else if (ls->pending) {
+      sh.nuc = 1;
+      sh.i1 = ls->pend_tgt;
+      sh.dt = 0.0;
       s_go = 2;
     } else if (events == 0 && ls->Na == ls->nmushy0) {
       s_go = 0;  // loop condition vSind2 != countS fails at once (S/VoxelsCA.C:381)
     }
   }
   __syncthreads();
-  while (s_go) {
-    if (s_go == 1) {
-      // ---- scan: every thread takes slots tid, tid+NT, ...
-      Best b;
-      b.t = __builtin_inf();
-      b.r = __builtin_inf();
-      b.key = 0x7fffffffffffffffll;
-      b.cand = -1;
-      const int ns = s_nslots;
-      for (int s = tid; s < ns; s += EL_NT) {
-        if (d.gDead[s]) continue;
-        visits++;
-        const double E = d.gExt[s], V = d.gVhat[s];
-        const int n = d.gN[s];
-        const size_t cb = (size_t)s * AMCD_NB;
-        const long long kb = (long long)d.gScan[s] * 32;
-        bool live = false;
-        for (int q = 0; q < n; ++q) {
-          int tg = d.cTgt[cb + q];
-          if (d.aVS[tg] != 1) continue;
-          live = true;
-          evals++;
-          double dr = d.cD[cb + q] - E;
-          double t = dr / V;
-          if (!(t <= 1e6)) continue;
-          Best c;
-          c.t = t;
-          c.r = d.cR[cb + q];
-          c.key = kb + d.cK[cb + q];
-          c.cand = (int)(cb + q);
-          if (better(c, b)) b = c;
-        }
-        d.gGrow[s] = live;
-        if (!live) d.gDead[s] = 1;
-      }
-      for (int m = 32; m >= 1; m >>= 1) {
-        Best o = shfl_best(b, m);
-        if (better(o, b)) b = o;
-      }
-      if (lane == 0) sb[w] = b;
-      __syncthreads();
-      if (tid == 0) {
-        Best bb = sb[0];
-        for (int q = 1; q < EL_NW; ++q)
-          if (better(sb[q], bb)) bb = sb[q];
-        if (bb.cand < 0 || bb.t >= 1e6) {
-          s_go = 0;  // S/VoxelsCA.C:480
-        } else {
-          unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
-          rng = x2;
-          draws += 2;
-          double U = canonical(x1, x2);
-          s_cand = bb.cand;
-          s_slot = bb.cand / AMCD_NB;
-          s_tgt = d.cTgt[bb.cand];
-          s_nuc = U < p.rX;
-          s_dt = s_nuc ? 0.0 : bb.t;
-          events++;
-        }
-      }
-      __syncthreads();
-      if (!s_go) break;
-      if (!s_nuc) {
-        // ---- growth of every cell that was growing at this scan (S/VoxelsCA.C:511-516)
-        const double Dt = s_dt;
-        for (int s = tid; s < ns; s += EL_NT) {
-          if (!d.gGrow[s]) continue;
-          double e = d.gExt[s] + d.gVhat[s] * Dt;
-          d.gExt[s] = (e < 0.0) ? 0.0 : e;
-        }
-      }
-    }
-    // ---- capture or nucleation of voxel s_tgt (thread 0)
-    if (tid == 0) {
-      const int i1 = s_tgt;
-      const int v1 = d.aVox[i1];
-      s_newslot = 0;
-      if (s_nuc) {
-        if (pool_used >= st.n_pool || pool_used >= d.poolcap || nGrain >= d.gcap) {  // suspend; host refills
-          ls->pending = 1;
-          ls->pend_tgt = i1;
-          s_go = -1;
-        } else {
-          const double* P = d.pool + 9 * (size_t)pool_used;
-          nGrain += 1;
-          double* R = d.rot + 9 * (size_t)(nGrain - 1);
-          for (int q = 0; q < 9; ++q) R[q] = P[q];
-          pool_used++;
-          nucs++;
-          double c[3];
-          centre(d, v1, c);
-          d.aG[i1] = nGrain;
-          d.aCx[i1] = c[0];
-          d.aCy[i1] = c[1];
-          d.aCz[i1] = c[2];
-          d.aExt[i1] = 0.0;
-          ls->pending = 0;
-        }
-      } else {
-        const int js = d.gAct[s_slot];
-        const int G = min(max(d.aG[js], 1), d.gcap);  // valid by construction (checked at slot creation)
-        const double* R = d.rot + 9 * (size_t)(G - 1);
-        double cen[3] = {d.aCx[js], d.aCy[js], d.aCz[js]}, nc[3], ne;
-        recentre(d, d.aVox[js], v1, cen, R, d.cF[s_cand], nc, &ne);
-        d.aG[i1] = G;
-        d.aCx[i1] = nc[0];
-        d.aCy[i1] = nc[1];
-        d.aCz[i1] = nc[2];
-        d.aExt[i1] = ne;
-        captures++;
-      }
-      if (s_go != -1) {
-        d.aVS[i1] = 2;
-        d.aScan[i1] = vSind2;
-        vSind2++;
-        tinc += s_dt;
-        s_newslot = d.aCold[i1];
-      }
-    }
-    __syncthreads();
-    if (s_go == -1) break;
-    // ---- the new mushy voxel joins the growing list if it has liquid neighbours
-    if (s_newslot && w == 0) {
-      const int i1 = s_tgt;
-      int t = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
-      bool liq = t >= 0 && d.aVS[t] == 1;
-      unsigned long long bal = __ballot(liq);
-      if (bal) {
-        const int slot = s_nslots;
-        const size_t cb = (size_t)slot * AMCD_NB;
-        if (liq) {
-          unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-          int r = __popcll(bal & lt);
-          const int G = min(max(d.aG[i1], 1), d.gcap);
-          const double* Rg = d.rot + 9 * (size_t)(G - 1);
-          double Rl[9];
-          for (int q = 0; q < 9; ++q) Rl[q] = Rg[q];
-          double c[3] = {d.aCx[i1], d.aCy[i1], d.aCz[i1]};
-          double D, f;
-          candidate(d, c, Rl, d.aVox[t], &D, &f);
-          d.cD[cb + r] = D;
-          d.cF[cb + r] = f;
-          d.cR[cb + r] = .5 * (d.ruq[i1] + d.ruq[t]);
-          d.cTgt[cb + r] = t;
-          d.cK[cb + r] = (uint8_t)lane;
-        }
+  int go = s_go;
+  if (w > 0) {
+    owner_loop<NT, K>(d, sh, go, visits, evals);
+  } else {
+    while (go > 0) {
+      if (go == 1) {
+        __syncthreads();  // 1
         if (lane == 0) {
-          int err = 0;
-          d.gAct[slot] = i1;
-          d.gScan[slot] = d.aScan[i1];
-          d.gExt[slot] = d.aExt[i1];
-          d.gVhat[slot] = growth_velocity(p, d.aT[i1], &err);
-          d.gN[slot] = (uint8_t)__popcll(bal);
-          d.gDead[slot] = 0;
-          if (err) atomicOr(&O[blockIdx.x].err_nonanalytic, 1);
-          s_nslots = slot + 1;
+          Best bb = sh.sb[1];
+          for (int q = 2; q < NW; ++q)
+            if (better(sh.sb[q], bb)) bb = sh.sb[q];
+          sh.stop = 0;
+          if (bb.cand < 0 || bb.t >= 1e6) {
+            sh.stop = 1;  // S/VoxelsCA.C:480
+          } else {
+            unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
+            rng = x2;
+            draws += 2;
+            double U = canonical(x1, x2);
+            sh.cand = bb.cand;
+            sh.i1 = d.cTgt[bb.cand];
+            sh.nuc = U < p.rX;
+            sh.dt = sh.nuc ? 0.0 : bb.t;
+            events++;
+          }
+        }
+        __syncthreads();  // 2
+        if (sh.stop) {
+          go = 0;
+          break;
         }
       }
+      // capture or nucleation of voxel sh.i1 (lane 0)
+      if (lane == 0) {
+        const int i1 = sh.i1;
+        const int v1 = d.aVox[i1];
+        sh.newslot = 0;
+        sh.next = 1;
+        if (sh.nuc) {
+          if (pool_used >= st.n_pool || pool_used >= d.poolcap || nGrain >= d.gcap) {
+            ls->pending = 1;  // the host refills the pool and resumes
+            ls->pend_tgt = i1;
+            sh.next = -1;
+          } else {
+            const double* P = d.pool + 9 * (size_t)pool_used;
+            nGrain += 1;
+            double* R = d.rot + 9 * (size_t)(nGrain - 1);
+            for (int q = 0; q < 9; ++q) R[q] = P[q];
+            pool_used++;
+            nucs++;
+            double c[3];
+            centre(d, v1, c);
+            d.aG[i1] = nGrain;
+            d.aCx[i1] = c[0];
+            d.aCy[i1] = c[1];
+            d.aCz[i1] = c[2];
+            d.aExt[i1] = 0.0;
+            ls->pending = 0;
+          }
+        } else {
+          const int js = d.gAct[sh.cand / AMCD_NB];
+          const int G = min(max(d.aG[js], 1), d.gcap);
+          const double* R = d.rot + 9 * (size_t)(G - 1);
+          double cen[3] = {d.aCx[js], d.aCy[js], d.aCz[js]}, nc[3], ne;
+          recentre(d, d.aVox[js], v1, cen, R, d.cF[sh.cand], nc, &ne);
+          d.aG[i1] = G;
+          d.aCx[i1] = nc[0];
+          d.aCy[i1] = nc[1];
+          d.aCz[i1] = nc[2];
+          d.aExt[i1] = ne;
+          captures++;
+        }
+        if (sh.next != -1) {
+          d.aVS[i1] = 2;
+          d.aScan[i1] = vSind2;
+          vSind2++;
+          tinc += sh.dt;
+          sh.newslot = d.aCold[i1];
+          if (!(tinc < p.DelT)) sh.next = 0;  // loop condition, S/VoxelsCA.C:381
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      if (sh.newslot) {  // the new mushy voxel joins the growing list if it has liquid neighbours
+        const int i1 = sh.i1;
+        int t = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
+        bool liq = t >= 0 && d.aVS[t] == 1;
+        unsigned long long bal = __ballot(liq);
+        if (bal) {
+          const int slot = sh.nslots;
+          const size_t cb = (size_t)slot * AMCD_NB;
+          double D = __builtin_inf(), f = 0, r = __builtin_inf();
+          int key = 0x7fffffff;
+          const int kb = d.aScan[i1] * 32;
+          if (liq) {
+            const int G = min(max(d.aG[i1], 1), d.gcap);
+            const double* Rg = d.rot + 9 * (size_t)(G - 1);
+            double Rl[9];
+            for (int q = 0; q < 9; ++q) Rl[q] = Rg[q];
+            double c[3] = {d.aCx[i1], d.aCy[i1], d.aCz[i1]};
+            candidate(d, c, Rl, d.aVox[t], &D, &f);
+            r = .5 * (d.ruq[i1] + d.ruq[t]);
+            key = kb + lane;
+          }
+          int rank = 0;  // position in (D, ruqs, key) order
+          for (int j = 0; j < AMCD_NB; ++j) {
+            double Dj = __shfl(D, j), rj = __shfl(r, j);
+            int kj = __shfl(key, j);
+            if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
+          }
+          if (liq) {
+            d.cD[cb + rank] = D;
+            d.cF[cb + rank] = f;
+            d.cR[cb + rank] = r;
+            d.cTgt[cb + rank] = t;
+            d.cK[cb + rank] = (uint8_t)lane;
+            if (rank == 0) {
+              sh.mD1 = D;
+              sh.mR1 = r;
+              sh.mkey1 = key;
+              sh.mc1 = (int)cb;
+              sh.mtgt1 = t;
+            }
+            if (rank == 1) sh.mD2 = D;
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+          __builtin_amdgcn_wave_barrier();
+          if (lane == 0) {
+            int err = 0;
+            if (__popcll(bal) < 2) sh.mD2 = __builtin_inf();
+            d.gAct[slot] = i1;
+            d.gScan[slot] = d.aScan[i1];
+            d.gExt[slot] = d.aExt[i1];
+            d.gVhat[slot] = growth_velocity(p, d.aT[i1], &err);
+            d.gN[slot] = (uint8_t)__popcll(bal);
+            d.gDead[slot] = 0;
+            if (err) atomicOr(&O[blockIdx.x].err_nonanalytic, 1);
+            sh.mE = d.gExt[slot];
+            sh.mV = d.gVhat[slot];
+            sh.nslots = slot + 1;
+            if (slot < CAP) {
+              sh.newslot = 2;  // an owner takes it into registers
+            } else {
+              sh.newslot = 0;
+              if (sh.next > 0) sh.next = -2;  // beyond this tier: resume with a larger one
+            }
+          }
+        } else if (lane == 0) {
+          sh.newslot = 0;
+        }
+      }
+      __syncthreads();  // 3
+      go = sh.next;
     }
-    if (tid == 0) {
-      if (!(tinc < p.DelT)) s_go = 0;  // loop condition, S/VoxelsCA.C:381
-      else s_go = 1;
-    }
-    __syncthreads();
   }
   __syncthreads();
-  // ---- write back the growing cells' half-diagonals and the loop state
-  const int ns = s_nslots;
-  const bool suspended = (s_go == -1);
-  if (!suspended)
-    for (int s = tid; s < ns; s += EL_NT) d.aExt[d.gAct[s]] = d.gExt[s];
+  // ---- loop state and counters
+  const int ns = sh.nslots;
+  const bool suspended = go < 0;
+  if (!suspended && go != -2)
+    for (int s = tid; s < ns; s += NT) d.aExt[d.gAct[s]] = d.gExt[s];
   atomicAdd((unsigned long long*)&O[blockIdx.x].slot_visits, (unsigned long long)visits);
   atomicAdd((unsigned long long*)&O[blockIdx.x].cand_evals, (unsigned long long)evals);
   if (tid == 0) {
@@ -853,7 +1049,7 @@ __global__ __launch_bounds__(EL_NT) void k_event_loop(Inst* I, const amcd_step* 
     ls->nslots = ns;
     ls->done = suspended ? 0 : 1;
     amcd_out& o = O[blockIdx.x];
-    o.status = suspended ? 1 : 0;
+    o.status = go == -1 ? 1 : (go == -2 ? 2 : 0);
     o.n_events = events;
     o.n_captures = captures;
     o.n_nucleations = nucs;
@@ -965,6 +1161,7 @@ struct amcd_batch {
   hipEvent_t ev[16];
   double last_ms[16] = {0};
   bool timing = false;
+  int tier = 1;  // capture-loop capacity tier (see launch_event_loop)
 };
 
 static thread_local std::string g_dev_err;
@@ -1288,6 +1485,30 @@ static int launch_update(amcd_batch* b, bool fused_temperature) {
   return 0;
 }
 
+// Capacity tiers of the capture loop: threads per workgroup x growing cells per thread.
+static int launch_event_loop(amcd_batch* b) {
+  const int n = b->n;
+  switch (b->tier) {
+    case 0:
+      hipLaunchKernelGGL((k_event_loop<256, 4>), dim3(n), dim3(256), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_event_loop<512, 4>), dim3(n), dim3(512), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_event_loop<512, 8>), dim3(n), dim3(512), 0, b->stream, b->d_inst, b->d_steps,
+                         b->d_outs);
+      break;
+    default:
+      hipLaunchKernelGGL((k_event_loop<512, 16>), dim3(n), dim3(512), 0, b->stream, b->d_inst, b->d_steps,
+                         b->d_outs);
+      break;
+  }
+  HIPT(hipGetLastError());
+  return 0;
+}
+static const int kTierCap[4] = {192 * 4, 448 * 4, 448 * 8, 448 * 16};
+
 static int read_outs(amcd_batch* b, amcd_out* outs) {
   HIPT(hipMemcpyAsync(b->h_outs, b->d_outs, sizeof(amcd_out) * b->n, hipMemcpyDeviceToHost, b->stream));
   HIPT(hipStreamSynchronize(b->stream));
@@ -1306,7 +1527,7 @@ static int launch_active(amcd_batch* b) {
   hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
                      b->d_outs);
   tmark(b, 4);
-  hipLaunchKernelGGL(k_event_loop, dim3(n), dim3(EL_NT), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
+  if (launch_event_loop(b)) return -1;
   tmark(b, 5);
   HIPT(hipGetLastError());
   return 0;
@@ -1355,6 +1576,30 @@ static int grow_active(amcd_batch* b, const amcd_out* outs) {
   return 0;
 }
 
+// Relaunch (resume) the capture loop with a larger tier while some instance has more
+// growing cells than the current tier holds.
+static int regrow_tier(amcd_batch* b, amcd_out* outs) {
+  for (;;) {
+    bool need = false;
+    for (int i = 0; i < b->n; ++i) need = need || outs[i].status == 2;
+    if (!need) return 0;
+    if (b->tier >= 3) {
+      g_dev_err = "more growing cells than the largest capture-loop tier holds";
+      return -1;
+    }
+    b->tier++;
+    if (launch_event_loop(b) || read_outs(b, outs)) return -1;
+  }
+}
+// smallest tier holding 1.5x the largest growing list of the step just done
+static void pick_tier(amcd_batch* b, const amcd_out* outs) {
+  int mx = 0;
+  for (int i = 0; i < b->n; ++i) mx = std::max(mx, outs[i].n_slots);
+  int t = 0;
+  while (t < 3 && kTierCap[t] < mx + mx / 2) ++t;
+  b->tier = t;
+}
+
 static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, bool fused) {
   if (put_steps(b, steps)) return -1;
   for (int attempt = 0; attempt < 8; ++attempt) {
@@ -1370,12 +1615,14 @@ static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, 
     if (!ovf) break;
     if (grow_active(b, outs)) return -1;
   }
+  if (regrow_tier(b, outs)) return -1;
   bool pend = false;
   for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
   if (!pend) {
     if (launch_finish(b)) return -1;
     HIPT(hipStreamSynchronize(b->stream));
     collect_timing(b);
+    pick_tier(b, outs);
   }
   return 0;
 }
@@ -1390,9 +1637,9 @@ int amcd_step_fused(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
 
 int amcd_resume(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
   if (put_steps(b, steps)) return -1;
-  hipLaunchKernelGGL(k_event_loop, dim3(b->n), dim3(EL_NT), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
-  HIPT(hipGetLastError());
+  if (launch_event_loop(b)) return -1;
   if (read_outs(b, outs)) return -1;
+  if (regrow_tier(b, outs)) return -1;
   bool pend = false;
   for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
   if (!pend) {
