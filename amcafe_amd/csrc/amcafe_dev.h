@@ -106,6 +106,8 @@ int amcd_reserve_grains(amcd_batch* b, int i, int need);
 void amcd_set_timing(amcd_batch* b, int on);
 int amcd_last_timings(amcd_batch* b, double* ms, int n);
 const char* amcd_last_error(void);
+// capture-loop phase timers of a -DAMCD_PHASE_PROFILE build (zeros otherwise); resets them
+int amcd_phase_profile(unsigned long long* out16);
 
 // known-answer helpers: libstdc++ uniform_real_distribution<double> over
 // minstd_rand0(seed), sequentially and through the device's jump-ahead path

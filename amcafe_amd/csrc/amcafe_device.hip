@@ -32,6 +32,22 @@
 #include "amcafe_dev.h"
 #include "ddmath.h"
 
+// Phase timers of the capture loop (core-clock cycles summed over events), compiled in
+// only with -DAMCD_PHASE_PROFILE; read with amcd_phase_profile().
+#ifdef AMCD_PHASE_PROFILE
+__device__ unsigned long long g_phase[16];
+#define PHASE_INIT(v) unsigned long long v = clock64()
+#define PHASE(v, slot, pred)                                \
+  do {                                                      \
+    unsigned long long t_ = clock64();                      \
+    if (pred) atomicAdd(&g_phase[slot], t_ - v);            \
+    v = t_;                                                 \
+  } while (0)
+#else
+#define PHASE_INIT(v)
+#define PHASE(v, slot, pred)
+#endif
+
 #define ZB 1024     // voxels per compaction block (one per thread)
 #define GB 256      // threads per grid-wide block
 
@@ -107,6 +123,11 @@ __host__ __device__ static inline unsigned int lcg_pow(unsigned long long e) {  
     e >>= 1;
   }
   return (unsigned int)r;
+}
+// two steps back: 16807^-2 mod (2^31-1) = 1407677000^2 mod m (16807 * 1407677000 = 1 mod m)
+__host__ __device__ static inline unsigned int rng_back(unsigned int x) {
+  const unsigned long long inv = 1407677000ull;
+  return mod_m(mod_m((unsigned long long)x * inv) * inv);
 }
 __host__ __device__ static inline unsigned int lcg_seed(unsigned long long s) {
   unsigned int x = (unsigned int)(s % MINSTD_M);
@@ -415,7 +436,7 @@ __device__ static inline void centre(const Inst& d, int v, double* c) {
 
 // Capture candidate of growing cell a (centroid c, rotation R) towards liquid voxel
 // vt: D = pow(cos th sin ph,.5) * |R^T dnx|_1 and the factor f (S/VoxelsCA.C:409-424).
-__device__ __noinline__ static void candidate(const Inst& d, const double* c, const double* R, int vt, double* D,
+__device__ static inline void candidate(const Inst& d, const double* c, const double* R, int vt, double* D,
                                         double* f) {
   double ct[3];
   centre(d, vt, ct);
@@ -560,7 +581,7 @@ __device__ static inline void project_point_line(const double* A, const double* 
 __device__ static inline double dist3(const double* p, const double* q) {
   return sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) + (p[2] - q[2]) * (p[2] - q[2]));
 }
-__device__ __noinline__ static void recentre(const Inst& d, int vcap, int vnew, const double* cen, const double* R, double f,
+__device__ static inline void recentre(const Inst& d, int vcap, int vnew, const double* cen, const double* R, double f,
                                 double* ncen, double* next) {
   const amcd_params& p = d.p;
   int jx0, jx1, jx2, jy0, jy1, jy2;
@@ -625,17 +646,49 @@ __device__ __noinline__ static void recentre(const Inst& d, int vcap, int vnew, 
   ncen[2] = cen[2] + locX[2];
 }
 
+// The capture / nucleation loop of one instance (S/VoxelsCA.C:381-610): one workgroup
+// of NT threads per instance, in two roles.
+//   wave 0      the engine: performs each event (nucleation, or capture with octahedron
+//               re-centring, S/VoxelsCA.C:482-605) and builds the sorted candidate list
+//               of the newly mushy voxel;
+//   waves 1..   the owners: thread 64+j owns growing cells j, j+(NT-64), ... (K of them)
+//               and keeps each cell's half-diagonal E, velocity V, the head of its
+//               sorted candidate list and float bounds on its capture time in registers.
+// An event is EVAL | barrier | DECIDE | APPLY:
+//   EVAL    each owner wave finds its exact best candidate (t, ruqs, scan order) among
+//           the few cells whose time bounds reach the wave's smallest upper bound; the
+//           engine has already placed the best candidate of the cell it created last;
+//   DECIDE  every wave reads the per-wave bests and the pre-drawn uniform and reaches
+//           the same decision independently, so no second barrier is needed;
+//   APPLY   the owners grow their cells by V*Dt, take over the cell created in the
+//           previous event and re-read lists whose first live target was just captured,
+//           while the engine performs the event and evaluates the new cell.
+// The exchange areas are double-buffered by event parity: one barrier per event keeps
+// every wave within one event of the others.
 struct Best {
   double t, r;
-  int key;
-  int cand;
+  int key;   // scan position * 32 + neighbour rank: the reference's scan order
+  int cand;  // candidate entry (slot * AMCD_NB + position)
+  int act;   // active index of the capturing cell
+  int tgt;   // active index of the captured voxel
 };
-__device__ static inline bool better(const Best& a, const Best& b) {  // a strictly before b
-  if (a.t < b.t) return true;
-  if (a.t > b.t) return false;
-  if (a.r < b.r) return true;
-  if (a.r > b.r) return false;
-  return a.key < b.key;
+__device__ static inline Best best_none() {
+  Best b;
+  b.t = __builtin_inf();
+  b.r = __builtin_inf();
+  b.key = 0x7fffffff;
+  b.cand = -1;
+  b.act = -1;
+  b.tgt = -1;
+  return b;
+}
+// a strictly before b in (t, ruqs, scan order).  The ordering predicates are written
+// without short-circuit branches: hipcc (ROCm 7.2, gfx950, -O3) miscompiles
+// `if (better(o, b)) b = o;` when better() returns early from a chain of comparisons,
+// leaving some fields of b from the old value and some from o.
+__device__ static inline bool better(const Best& a, const Best& b) {
+  const bool tl = a.t < b.t, tg = a.t > b.t, rl = a.r < b.r, rg = a.r > b.r;
+  return tl | (!tg & (rl | (!rg & (a.key < b.key))));
 }
 __device__ static inline Best shfl_best(const Best& b, int lane_mask) {
   Best o;
@@ -643,192 +696,312 @@ __device__ static inline Best shfl_best(const Best& b, int lane_mask) {
   o.r = __shfl_xor(b.r, lane_mask);
   o.key = __shfl_xor(b.key, lane_mask);
   o.cand = __shfl_xor(b.cand, lane_mask);
+  o.act = __shfl_xor(b.act, lane_mask);
+  o.tgt = __shfl_xor(b.tgt, lane_mask);
   return o;
 }
+// best over a wave; a wave with at most one candidate needs no shuffles
+__device__ static inline Best wave_best(Best b) {
+  const unsigned long long m = __ballot(b.cand >= 0);
+  if (m == 0) return best_none();
+  if (__popcll(m) == 1) {
+    const int src = __builtin_ctzll(m);
+    Best o;
+    o.t = __shfl(b.t, src);
+    o.r = __shfl(b.r, src);
+    o.key = __shfl(b.key, src);
+    o.cand = __shfl(b.cand, src);
+    o.act = __shfl(b.act, src);
+    o.tgt = __shfl(b.tgt, src);
+    return o;
+  }
+  for (int s = 32; s >= 1; s >>= 1) {
+    Best o = shfl_best(b, s);
+    if (better(o, b)) b = o;
+  }
+  return b;
+}
 // candidate order inside a growing cell's list: (D, ruqs, key), NaN distances last
+// (branch-free for the reason given at better())
 __device__ static inline bool cand_less(double Da, double Ra, int ka, double Db, double Rb, int kb) {
-  bool na = Da != Da, nb = Db != Db;
-  if (na != nb) return nb;
-  if (!na && Da != Db) return Da < Db;
-  if (Ra != Rb) return Ra < Rb;
-  return ka < kb;
+  const bool na = Da != Da, nb = Db != Db;
+  const bool c1 = na != nb;             // exactly one NaN: the other one is first
+  const bool c2 = !na & !nb & (Da != Db);  // both numbers, different
+  const bool rne = Ra != Rb;
+  const bool tail = (rne & (Ra < Rb)) | (!rne & (ka < kb));
+  return (c1 & nb) | (!c1 & c2 & (Da < Db)) | (!c1 & !c2 & tail);
 }
 
-// First live entry at or after position `from` of slot s's sorted list, and the
-// distance of the next live entry (+inf if none).  Entries before the first live one
-// stay dead: a captured voxel never becomes liquid again within the step.
+// First live entry at or after position `from` of slot s's sorted list.  Entries
+// before the first live one stay dead: a captured voxel never becomes liquid again
+// within the step.  `skip` is a voxel that counts as dead although its state may not
+// be written yet.
 struct SlotHead {
-  double D1, R1, D2;
-  int key1, c1, tgt1;
-  bool alive;
+  double D1;
+  int c1, tgt1;  // tgt1 = -1: no live entry
 };
 __device__ static inline bool live_target(const Inst& d, int t, int skip) { return t != skip && d.aVS[t] == 1; }
-__device__ __noinline__ static void slot_head(const Inst& d, int s, int from, int skip, SlotHead& h) {
+__device__ static inline SlotHead slot_head(const Inst& d, int s, int from, int skip) {
+  SlotHead h;
   const int n = d.gN[s];
   const size_t cb = (size_t)s * AMCD_NB;
   int p = from;
   while (p < n && !live_target(d, d.cTgt[cb + p], skip)) ++p;
-  if (p >= n) {
-    h.alive = false;
-    h.D1 = h.D2 = __builtin_inf();
-    h.R1 = 0;
-    h.key1 = 0;
-    h.c1 = (int)cb + n;
-    h.tgt1 = -1;
-    return;
-  }
-  h.alive = true;
-  h.c1 = (int)(cb + p);
-  h.tgt1 = d.cTgt[cb + p];
-  h.D1 = d.cD[cb + p];
-  h.R1 = d.cR[cb + p];
-  h.key1 = d.gScan[s] * 32 + d.cK[cb + p];
-  int q = p + 1;
-  while (q < n && !live_target(d, d.cTgt[cb + q], skip)) ++q;
-  h.D2 = (q < n) ? d.cD[cb + q] : __builtin_inf();
+  h.c1 = (int)cb + p;
+  h.tgt1 = (p < n) ? d.cTgt[cb + p] : -1;
+  h.D1 = (p < n) ? d.cD[cb + p] : __builtin_inf();
+  return h;
 }
 
-// Exact best of a slot when its first two live entries tie in time: walk the block of
-// live entries with the same time (times are non-decreasing along the sorted list).
-__device__ __noinline__ static Best slot_walk(const Inst& d, int s, int c1, double E, double V, Best b, long long& evals) {
-  const int n = d.gN[s];
-  const size_t cb = (size_t)s * AMCD_NB;
-  const int kb = d.gScan[s] * 32;
-  for (int p = c1 - (int)cb + 1; p < n; ++p) {
-    if (d.aVS[d.cTgt[cb + p]] != 1) continue;
-    evals++;
-    double t = (d.cD[cb + p] - E) / V;
-    if (!(t == b.t)) break;
-    Best c;
-    c.t = t;
-    c.r = d.cR[cb + p];
-    c.key = kb + d.cK[cb + p];
-    c.cand = (int)(cb + p);
-    if (better(c, b)) b = c;
+// Float bounds [lo, hi] on tau = T + t, the event clock at which the cell's head would
+// be captured, t = (D1 - E)/V.  While the head does not change, every event adds Dt to
+// T and V*Dt to E, so tau stays fixed up to rounding; the margin, 2^-26 of the
+// magnitudes involved, exceeds that drift for more than 2^20 events (the loop
+// re-evaluates every cell before that).  lo = hi = +inf: never a candidate (t NaN or
+// +inf); lo = -inf, hi = +inf: evaluated at every event (V <= 0 or t = -inf).
+__device__ static inline float f_down(double x) {
+  float f = (float)x;
+  return ((double)f > x) ? nextafterf(f, -__builtin_inff()) : f;
+}
+__device__ static inline float f_up(double x) {
+  float f = (float)x;
+  return ((double)f < x) ? nextafterf(f, __builtin_inff()) : f;
+}
+__device__ static inline void tau_bounds(double D1, double e, double v, double T, float& lo, float& hi) {
+  const double t = (D1 - e) / v;
+  if (t != t || t == __builtin_inf()) {
+    lo = hi = __builtin_inff();
+  } else if (!(v > 0.0) || t == -__builtin_inf()) {
+    lo = -__builtin_inff();
+    hi = __builtin_inff();
+  } else {
+    const double m = 0x1p-26 * (fabs(t) + fabs(T) + (fabs(D1) + fabs(e)) / v);
+    const double tau = T + t;
+    lo = f_down(tau - m);
+    hi = f_up(tau + m);
   }
-  return b;
 }
 
-// The capture / nucleation loop of one instance (S/VoxelsCA.C:381-610), one workgroup
-// of NT threads per instance, split in two roles that meet at three barriers per
-// iteration:
-//   wave 0      the engine: picks the winner of the argmin, draws the uniform,
-//               nucleates or captures with octahedron re-centring, and builds the
-//               sorted candidate list of the newly mushy voxel;
-//   waves 1..   the owners: thread 64+j owns growing cells j, j+(NT-64), ... (K of
-//               them) and keeps each cell's half-diagonal E, velocity V and the head
-//               of its sorted candidate list in registers.
-// Per iteration the owners compute every cell's capture time t = (D1 - E)/V of its
-// first live candidate (the minimum: t is non-decreasing along a list sorted by D;
-// exact ties in t are resolved by walking the list), reduce (t, ruqs, scan order),
-// then, while the engine works on the captured voxel, grow every cell by V*Dt and
-// re-read the lists whose first live target was captured.
-struct EngineShared {
-  Best sb[16];
-  int stop, nuc, i1, cand, nslots, next, newslot;
-  double dt;
-  double mE, mV, mD1, mR1, mD2;
-  int mkey1, mc1, mtgt1;
+struct NewCell {  // a growing cell created by the engine, taken over by its owner
+  double E, V, D1;
+  int c1, tgt1, slot, valid;
 };
+struct EngineShared {
+  Best sb[2][16];   // per-wave bests, by event parity
+  double nextU[2];  // the uniform of the next event, by parity
+  int susp[2];      // engine suspension request (-1 pool refill, -2 larger tier), by parity
+  NewCell mb[2];    // cell created in the event of this parity
+  int nslots;
+  int r_i1;         // resume: pending nucleation target
+  int fin;          // exit code of the loop
+};
+// The decision every wave takes after the barrier (S/VoxelsCA.C:476-485).
+struct Decision {
+  int stop, nuc, i1, cand, js;
+  double dt;
+};
+template <int NW>
+__device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX) {
+  Best b = sh.sb[par][0];
+  for (int q = 1; q < NW; ++q) {
+    const Best o = sh.sb[par][q];
+    if (better(o, b)) b = o;
+  }
+  Decision x;
+  x.stop = 0;
+  x.nuc = 0;
+  x.i1 = b.tgt;
+  x.cand = b.cand;
+  x.js = b.act;
+  x.dt = 0.0;
+  if (!(T < DelT)) {
+    x.stop = 1;  // loop condition (S/VoxelsCA.C:381)
+  } else if (sh.susp[par]) {
+    x.stop = sh.susp[par];
+  } else if (b.cand < 0 || b.t >= 1e6) {
+    x.stop = 1;  // S/VoxelsCA.C:480
+  } else {
+    x.nuc = sh.nextU[par] < rX;
+    x.dt = x.nuc ? 0.0 : b.t;
+  }
+  return x;
+}
 
 template <int NT, int K>
-__device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, long long& visits, long long& evals) {
+__device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, double T, long long& visits,
+                                  long long& evals) {
   constexpr int NO = NT - 64;  // owner threads
+  constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int me = tid - 64;
-  double E[K], V[K], D1[K], R1[K], D2[K];
-  int key1[K], c1[K], tgt1[K];
-  bool alive[K];
+  const float fInf = __builtin_inff();
+  const amcd_params& p = d.p;
+  double E[K], V[K];
+  float lo[K], hi[K];
+  int c1[K], tgt1[K];
   const bool loaded = go != -2;
   const int ns0 = sh.nslots;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int s = me + k * NO;
-    alive[k] = false;
-    E[k] = V[k] = D1[k] = R1[k] = D2[k] = 0;
-    key1[k] = c1[k] = tgt1[k] = 0;
-    if (s < ns0 && loaded) {
+    E[k] = V[k] = 0.0;
+    lo[k] = hi[k] = fInf;
+    c1[k] = 0;
+    tgt1[k] = -1;
+    if (loaded && s < ns0) {
       E[k] = d.gExt[s];
       V[k] = d.gVhat[s];
-      SlotHead h;
-      slot_head(d, s, 0, -1, h);
-      D1[k] = h.D1; R1[k] = h.R1; D2[k] = h.D2; key1[k] = h.key1; c1[k] = h.c1; tgt1[k] = h.tgt1;
-      alive[k] = h.alive;
+      const SlotHead h = slot_head(d, s, 0, -1);
+      c1[k] = h.c1;
+      tgt1[k] = h.tgt1;
+      if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
     }
   }
+  int skip = -1;  // voxel the engine may be capturing right now
+  int par = 0;
+  int nev = 0;    // events since the bounds were last recomputed
+  Decision x;
+  x.stop = 0;
+  x.nuc = 1;
+  x.i1 = sh.r_i1;
+  x.dt = 0.0;
+  bool apply_first = go == 2;
   while (go > 0) {
-    if (go == 1) {
-      Best b;
-      b.t = __builtin_inf();
-      b.r = __builtin_inf();
-      b.key = 0x7fffffff;
-      b.cand = -1;
+    // The slot numbers are re-derived from an opaque copy of the thread index in every
+    // iteration: otherwise the compiler keeps per-cell addresses live across the loop,
+    // which costs more registers than the cell state itself.
+    int mo = me;
+    asm volatile("" : "+v"(mo));
+    if (!apply_first) {
+      // ---- EVAL
+      float u = fInf;
+#pragma unroll
+      for (int k = 0; k < K; ++k) u = fminf(u, hi[k]);
+      for (int m = 32; m >= 1; m >>= 1) u = fminf(u, __shfl_xor(u, m));
+      Best b = best_none();
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        if (!alive[k]) continue;
-        visits++;
-        evals++;
-        const double t1 = (D1[k] - E[k]) / V[k];
-        if (!(t1 <= 1e6)) continue;
-        Best c;
-        c.t = t1;
-        c.r = R1[k];
-        c.key = key1[k];
-        c.cand = c1[k];
-        const double t2 = (D2[k] - E[k]) / V[k];
-        if (t2 == t1) c = slot_walk(d, me + k * NO, c1[k], E[k], V[k], c, evals);
-        if (better(c, b)) b = c;
-      }
-      for (int m = 32; m >= 1; m >>= 1) {
-        Best o = shfl_best(b, m);
-        if (better(o, b)) b = o;
-      }
-      if (lane == 0) sh.sb[w] = b;
-      __syncthreads();  // 1: candidates in
-      __syncthreads();  // 2: decision out
-      if (sh.stop) break;
-      const int i1 = sh.i1;
-      if (!sh.nuc) {  // growth of every cell that was growing at this scan (S/VoxelsCA.C:511-516)
-        const double Dt = sh.dt;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          if (!alive[k]) continue;
-          double e = E[k] + V[k] * Dt;
-          E[k] = (e < 0.0) ? 0.0 : e;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {  // lists whose first live target is being captured now
-        if (alive[k] && tgt1[k] == i1) {
-          const int s = me + k * NO;
-          SlotHead h;
-          slot_head(d, s, c1[k] - s * AMCD_NB + 1, i1, h);
-          D1[k] = h.D1; R1[k] = h.R1; D2[k] = h.D2; key1[k] = h.key1; c1[k] = h.c1; tgt1[k] = h.tgt1;
-          alive[k] = h.alive;
-        }
-      }
-    }
-    __syncthreads();  // 3: capture done, new cell in the mailbox
-    if (sh.newslot == 2) {
-      const int slot = sh.nslots - 1;
-      if (slot % NO == me) {
-        const int kk = slot / NO;
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-          if (k == kk) {
-            E[k] = sh.mE; V[k] = sh.mV; D1[k] = sh.mD1; R1[k] = sh.mR1; D2[k] = sh.mD2;
-            key1[k] = sh.mkey1; c1[k] = sh.mc1; tgt1[k] = sh.mtgt1; alive[k] = true;
+        if (lo[k] != fInf && lo[k] <= u) {
+          // exact capture time of the cell's head (S/VoxelsCA.C:420-440)
+          const int s = mo + k * NO;
+          const int c = c1[k];
+          const int pos = c - s * AMCD_NB;
+          const double e = E[k], v = V[k];
+          const double D1 = d.cD[c];
+          const double Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
+          const double rr = d.cR[c];
+          const int kb = d.gScan[s] * 32;
+          const int kk = d.cK[c];
+          const int a = d.gAct[s];
+          visits++;
+          evals++;
+          const double t1 = (D1 - e) / v;
+          if (t1 <= 1e6) {
+            Best y;
+            y.t = t1;
+            y.r = rr;
+            y.key = kb + kk;
+            y.cand = c;
+            y.act = a;
+            y.tgt = tgt1[k];
+            // times are non-decreasing along the list: a tie needs the next entry's
+            // time to equal t1; then walk the tied block
+            if ((Dn - e) / v == t1) {
+              const int n = d.gN[s];
+              for (int q = pos + 1; q < n; ++q) {
+                const int tq = d.cTgt[c - pos + q];
+                if (!live_target(d, tq, skip)) continue;
+                evals++;
+                const double tt = (d.cD[c - pos + q] - e) / v;
+                if (!(tt == t1)) break;
+                Best z;
+                z.t = tt;
+                z.r = d.cR[c - pos + q];
+                z.key = kb + d.cK[c - pos + q];
+                z.cand = c - pos + q;
+                z.act = a;
+                z.tgt = tq;
+                if (better(z, y)) y = z;
+              }
+            }
+            if (better(y, b)) b = y;
           }
+        }
+      }
+      b = wave_best(b);
+      if (lane == 0) sh.sb[par][w] = b;
+      __syncthreads();
+      // ---- DECIDE
+      x = decide<NW>(sh, par, T, p.DelT, p.rX);
+      if (x.stop) break;
+    }
+    apply_first = false;
+    // ---- APPLY
+    const double Dt = x.dt;
+    if (!x.nuc) {  // growth of every cell growing at this scan (S/VoxelsCA.C:511-516)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (tgt1[k] < 0) continue;
+        const double e = E[k] + V[k] * Dt;
+        if (e < 0.0) {
+          E[k] = 0.0;
+          lo[k] = -fInf;  // clamped: recompute the bounds below
+        } else {
+          E[k] = e;
+        }
       }
     }
-    go = sh.next;
+    T = T + Dt;  // the engine's tinc: same additions in the same order
+    const int i1 = x.i1;
+    // the cell created by the previous event (its best was evaluated by the engine)
+    const NewCell& nc = sh.mb[par];
+    if (nc.valid && nc.slot % NO == mo) {
+      const int kk = nc.slot / NO;
+      double e = nc.E;
+      if (!x.nuc) {
+        const double g = e + nc.V * Dt;
+        e = (g < 0.0) ? 0.0 : g;
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (k == kk) {
+          E[k] = e;
+          V[k] = nc.V;
+          c1[k] = nc.c1;
+          tgt1[k] = nc.tgt1;
+          lo[k] = -fInf;  // bounds below (its head may also be the voxel captured now)
+        }
+    }
+    // heads captured now, clamped cells, the new cell, and every cell once per 2^20 events
+    ++nev;
+    const bool all = nev >= (1 << 20);
+    if (all) nev = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool hit = tgt1[k] >= 0 && (tgt1[k] == i1 || lo[k] == -fInf || all);
+      if (!hit) continue;
+      const int s = mo + k * NO;
+      if (tgt1[k] == i1) {
+        const SlotHead h = slot_head(d, s, c1[k] - s * AMCD_NB + 1, i1);
+        c1[k] = h.c1;
+        tgt1[k] = h.tgt1;
+        lo[k] = hi[k] = fInf;
+        if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
+      } else {
+        tau_bounds(d.cD[c1[k]], E[k], V[k], T, lo[k], hi[k]);
+      }
+    }
+    skip = i1;
+    par ^= 1;
   }
+  // half-diagonals back to the growing list (the cell still in the mailbox is current there)
   if (loaded) {
-    const int ns = sh.nslots;  // stable: the engine has stopped
+    const int ns = sh.nslots;
+    const NewCell& nc = sh.mb[par];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int s = me + k * NO;
-      if (s < ns) d.gExt[s] = E[k];
+      if (s < ns && !(nc.valid && nc.slot == s)) d.gExt[s] = E[k];
     }
   }
 }
@@ -845,219 +1018,243 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const amcd_params& p = d.p;
   const amcd_step& st = S[blockIdx.x];
-  unsigned int rng = 0;
-  double tinc = 0;
-  int vSind2 = 0, nGrain = 0, pool_used = 0, events = 0, captures = 0, nucs = 0;
-  long long draws = 0, visits = 0, evals = 0;
   if (tid == 0) {
-    rng = ls->rng;
-    tinc = ls->tinc;
-    vSind2 = ls->vSind2;
-    nGrain = ls->nGrain;
-    pool_used = ls->pool_used;
-    events = ls->events;
-    captures = ls->captures;
-    nucs = ls->nucs;
-    draws = ls->draws;
     sh.nslots = ls->nslots;
+    sh.mb[0].valid = sh.mb[1].valid = 0;
+    sh.susp[0] = sh.susp[1] = 0;
+    for (int q = 0; q < 16; ++q) sh.sb[0][q] = sh.sb[1][q] = best_none();
+    sh.fin = 0;
+    sh.r_i1 = -1;
     s_go = 1;
     if (ls->nslots > CAP) {
       s_go = -2;  // more growing cells than this tier holds
     } else if (ls->pending) {
-      sh.nuc = 1;
-      sh.i1 = ls->pend_tgt;
-      sh.dt = 0.0;
-      s_go = 2;
-    } else if (events == 0 && ls->Na == ls->nmushy0) {
-      s_go = 0;  // loop condition vSind2 != countS fails at once (S/VoxelsCA.C:381)
+      sh.r_i1 = ls->pend_tgt;
+      s_go = 2;   // resume with the pending nucleation
+    } else if (ls->events == 0 && ls->Na == ls->nmushy0) {
+      s_go = 0;   // loop condition vSind2 != countS fails at once (S/VoxelsCA.C:381)
     }
   }
   __syncthreads();
-  int go = s_go;
+  const int go = s_go;
+  long long visits = 0, evals = 0;
   if (w > 0) {
-    owner_loop<NT, K>(d, sh, go, visits, evals);
+    owner_loop<NT, K>(d, sh, go, ls->tinc, visits, evals);
   } else {
+    // engine state, identical in every lane of wave 0
+    double tinc = ls->tinc;
+    int vSind2 = ls->vSind2, nGrain = ls->nGrain, pool_used = ls->pool_used;
+    int events = ls->events, captures = ls->captures, nucs = ls->nucs;
+    int nslots = ls->nslots;
+    // lane 1 draws each uniform one event ahead
+    unsigned int rng = ls->rng;
+    long long draws = ls->draws;
+    if (go > 0 && lane == 1) {
+      unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
+      sh.nextU[go == 2 ? 1 : 0] = canonical(x1, x2);
+      rng = x2;
+    }
+    int par = 0, fin = 0;
+    Decision x;
+    x.stop = 0;
+    x.nuc = 1;
+    x.i1 = sh.r_i1;
+    x.cand = -1;
+    x.js = -1;
+    x.dt = 0.0;
+    bool apply_first = go == 2;
+    PHASE_INIT(et);
     while (go > 0) {
-      if (go == 1) {
-        __syncthreads();  // 1
-        if (lane == 0) {
-          Best bb = sh.sb[1];
-          for (int q = 2; q < NW; ++q)
-            if (better(sh.sb[q], bb)) bb = sh.sb[q];
-          sh.stop = 0;
-          if (bb.cand < 0 || bb.t >= 1e6) {
-            sh.stop = 1;  // S/VoxelsCA.C:480
-          } else {
-            unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
-            rng = x2;
-            draws += 2;
-            double U = canonical(x1, x2);
-            sh.cand = bb.cand;
-            sh.i1 = d.cTgt[bb.cand];
-            sh.nuc = U < p.rX;
-            sh.dt = sh.nuc ? 0.0 : bb.t;
-            events++;
-          }
-        }
-        __syncthreads();  // 2
-        if (sh.stop) {
-          go = 0;
+      if (!apply_first) {
+        PHASE(et, 0, tid == 0);
+        __syncthreads();
+        PHASE(et, 1, tid == 0);
+        x = decide<NW>(sh, par, tinc, p.DelT, p.rX);
+        if (x.stop) {
+          fin = x.stop < 0 ? x.stop : 0;
           break;
         }
+        events++;
+        if (lane == 1) {  // the draw was consumed: prepare the next one
+          draws += 2;
+          unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
+          sh.nextU[par ^ 1] = canonical(x1, x2);
+          rng = x2;
+        }
+        PHASE(et, 2, tid == 0);
       }
-      // capture or nucleation of voxel sh.i1 (lane 0)
-      if (lane == 0) {
-        const int i1 = sh.i1;
+      apply_first = false;
+      // ---- APPLY: the event on voxel i1
+      const int i1 = x.i1;
+      NewCell& nc = sh.mb[par ^ 1];
+      int susp = 0;
+      Best nb = best_none();
+      if (lane == 0) nc.valid = 0;
+      if (x.nuc && (pool_used >= st.n_pool || pool_used >= d.poolcap || nGrain >= d.gcap)) {
+        if (lane == 0) {
+          ls->pending = 1;  // the host refills the pool and resumes
+          ls->pend_tgt = i1;
+        }
+        susp = -1;
+      } else {
+        // loads that depend only on i1 go first, so they arrive during the re-centring
+        const int tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
         const int v1 = d.aVox[i1];
-        sh.newslot = 0;
-        sh.next = 1;
-        if (sh.nuc) {
-          if (pool_used >= st.n_pool || pool_used >= d.poolcap || nGrain >= d.gcap) {
-            ls->pending = 1;  // the host refills the pool and resumes
-            ls->pend_tgt = i1;
-            sh.next = -1;
-          } else {
-            const double* P = d.pool + 9 * (size_t)pool_used;
-            nGrain += 1;
-            double* R = d.rot + 9 * (size_t)(nGrain - 1);
-            for (int q = 0; q < 9; ++q) R[q] = P[q];
-            pool_used++;
-            nucs++;
-            double c[3];
-            centre(d, v1, c);
-            d.aG[i1] = nGrain;
-            d.aCx[i1] = c[0];
-            d.aCy[i1] = c[1];
-            d.aCz[i1] = c[2];
-            d.aExt[i1] = 0.0;
-            ls->pending = 0;
-          }
-        } else {
-          const int js = d.gAct[sh.cand / AMCD_NB];
-          const int G = min(max(d.aG[js], 1), d.gcap);
-          const double* R = d.rot + 9 * (size_t)(G - 1);
-          double cen[3] = {d.aCx[js], d.aCy[js], d.aCz[js]}, nc[3], ne;
-          recentre(d, d.aVox[js], v1, cen, R, d.cF[sh.cand], nc, &ne);
-          d.aG[i1] = G;
-          d.aCx[i1] = nc[0];
-          d.aCy[i1] = nc[1];
-          d.aCz[i1] = nc[2];
-          d.aExt[i1] = ne;
+        const int cold = d.aCold[i1];
+        const double T1 = d.aT[i1];
+        const double rq1 = d.ruq[i1];
+        int vst = 0, vt = 0;
+        double rqt = 0.0;
+        if (tn >= 0) {
+          vst = d.aVS[tn];
+          vt = d.aVox[tn];
+          rqt = d.ruq[tn];
+        }
+        double R[9], cen[3], E0;
+        int G;
+        if (x.nuc) {  // S/VoxelsCA.C:486-506
+          const double* P = d.pool + 9 * (size_t)pool_used;
+          for (int q = 0; q < 9; ++q) R[q] = P[q];
+          G = nGrain + 1;
+          centre(d, v1, cen);
+          E0 = 0.0;
+          if (lane < 9) d.rot[9 * (size_t)(G - 1) + lane] = P[lane];
+          nGrain = G;
+          pool_used++;
+          nucs++;
+          if (lane == 0) ls->pending = 0;
+        } else {      // S/VoxelsCA.C:507-605
+          const int js = x.js;
+          G = min(max(d.aG[js], 1), d.gcap);
+          const double* Rg = d.rot + 9 * (size_t)(G - 1);
+          for (int q = 0; q < 9; ++q) R[q] = Rg[q];
+          const double c0[3] = {d.aCx[js], d.aCy[js], d.aCz[js]};
+          recentre(d, d.aVox[js], v1, c0, R, d.cF[x.cand], cen, &E0);
           captures++;
         }
-        if (sh.next != -1) {
+        const int scan = vSind2;
+        if (lane == 0) {
+          d.aG[i1] = G;
+          d.aCx[i1] = cen[0];
+          d.aCy[i1] = cen[1];
+          d.aCz[i1] = cen[2];
+          d.aExt[i1] = E0;
           d.aVS[i1] = 2;
-          d.aScan[i1] = vSind2;
-          vSind2++;
-          tinc += sh.dt;
-          sh.newslot = d.aCold[i1];
-          if (!(tinc < p.DelT)) sh.next = 0;  // loop condition, S/VoxelsCA.C:381
+          d.aScan[i1] = scan;
         }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-      __builtin_amdgcn_wave_barrier();
-      if (sh.newslot) {  // the new mushy voxel joins the growing list if it has liquid neighbours
-        const int i1 = sh.i1;
-        int t = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
-        bool liq = t >= 0 && d.aVS[t] == 1;
-        unsigned long long bal = __ballot(liq);
-        if (bal) {
-          const int slot = sh.nslots;
+        vSind2++;
+        tinc += x.dt;
+        PHASE(et, 3, tid == 0);
+        // the new mushy voxel joins the growing list if it has liquid neighbours
+        const bool liq = tn >= 0 && vst == 1;
+        const unsigned long long bal = __ballot(liq);
+        if (cold && bal) {
+          const int slot = nslots;
           const size_t cb = (size_t)slot * AMCD_NB;
+          int err = 0;
+          const double V0 = growth_velocity(p, T1, &err);
           double D = __builtin_inf(), f = 0, r = __builtin_inf();
           int key = 0x7fffffff;
-          const int kb = d.aScan[i1] * 32;
           if (liq) {
-            const int G = min(max(d.aG[i1], 1), d.gcap);
-            const double* Rg = d.rot + 9 * (size_t)(G - 1);
-            double Rl[9];
-            for (int q = 0; q < 9; ++q) Rl[q] = Rg[q];
-            double c[3] = {d.aCx[i1], d.aCy[i1], d.aCz[i1]};
-            candidate(d, c, Rl, d.aVox[t], &D, &f);
-            r = .5 * (d.ruq[i1] + d.ruq[t]);
-            key = kb + lane;
+            candidate(d, cen, R, vt, &D, &f);
+            r = .5 * (rq1 + rqt);
+            key = scan * 32 + lane;
           }
           int rank = 0;  // position in (D, ruqs, key) order
           for (int j = 0; j < AMCD_NB; ++j) {
-            double Dj = __shfl(D, j), rj = __shfl(r, j);
-            int kj = __shfl(key, j);
+            const double Dj = __shfl(D, j), rj = __shfl(r, j);
+            const int kj = __shfl(key, j);
             if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
           }
           if (liq) {
             d.cD[cb + rank] = D;
             d.cF[cb + rank] = f;
             d.cR[cb + rank] = r;
-            d.cTgt[cb + rank] = t;
+            d.cTgt[cb + rank] = tn;
             d.cK[cb + rank] = (uint8_t)lane;
             if (rank == 0) {
-              sh.mD1 = D;
-              sh.mR1 = r;
-              sh.mkey1 = key;
-              sh.mc1 = (int)cb;
-              sh.mtgt1 = t;
+              nc.D1 = D;
+              nc.c1 = (int)cb;
+              nc.tgt1 = tn;
             }
-            if (rank == 1) sh.mD2 = D;
+            // this cell's exact best for the next event (all its targets are liquid)
+            const double tq = (D - E0) / V0;
+            if (tq <= 1e6) {
+              nb.t = tq;
+              nb.r = r;
+              nb.key = key;
+              nb.cand = (int)(cb + rank);
+              nb.act = i1;
+              nb.tgt = tn;
+            }
           }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-          __builtin_amdgcn_wave_barrier();
+          nb = wave_best(nb);
+          visits++;
+          evals += __popcll(bal);
           if (lane == 0) {
-            int err = 0;
-            if (__popcll(bal) < 2) sh.mD2 = __builtin_inf();
             d.gAct[slot] = i1;
-            d.gScan[slot] = d.aScan[i1];
-            d.gExt[slot] = d.aExt[i1];
-            d.gVhat[slot] = growth_velocity(p, d.aT[i1], &err);
+            d.gScan[slot] = scan;
+            d.gExt[slot] = E0;
+            d.gVhat[slot] = V0;
             d.gN[slot] = (uint8_t)__popcll(bal);
             d.gDead[slot] = 0;
             if (err) atomicOr(&O[blockIdx.x].err_nonanalytic, 1);
-            sh.mE = d.gExt[slot];
-            sh.mV = d.gVhat[slot];
-            sh.nslots = slot + 1;
-            if (slot < CAP) {
-              sh.newslot = 2;  // an owner takes it into registers
-            } else {
-              sh.newslot = 0;
-              if (sh.next > 0) sh.next = -2;  // beyond this tier: resume with a larger one
-            }
+            nc.E = E0;
+            nc.V = V0;
+            nc.slot = slot;
+            nc.valid = slot < CAP;  // beyond this tier: resume with a larger one
           }
-        } else if (lane == 0) {
-          sh.newslot = 0;
+          nslots = slot + 1;
+          if (slot >= CAP && tinc < p.DelT) susp = -2;
         }
+        PHASE(et, 4, tid == 0);
       }
-      __syncthreads();  // 3
-      go = sh.next;
+      if (lane == 0) {
+        sh.sb[par ^ 1][0] = nb;
+        sh.susp[par ^ 1] = susp;
+        sh.nslots = nslots;
+      }
+      par ^= 1;
+    }
+    if (tid == 0) {
+      sh.fin = fin;
+      ls->tinc = tinc;
+      ls->vSind2 = vSind2;
+      ls->nGrain = nGrain;
+      ls->pool_used = pool_used;
+      ls->events = events;
+      ls->captures = captures;
+      ls->nucs = nucs;
+    }
+    if (tid == 1 && go != -2) {
+      // lane 1 holds the engine one uniform ahead: store the state before that draw
+      ls->rng = go > 0 ? rng_back(rng) : rng;
+      ls->draws = draws;
+      O[blockIdx.x].draws = draws;
     }
   }
   __syncthreads();
   // ---- loop state and counters
   const int ns = sh.nslots;
-  const bool suspended = go < 0;
-  if (!suspended && go != -2)
+  const int fin = sh.fin;  // 0 finished, -1 pool refill, -2 larger tier
+  const bool suspended = go == -2 || fin < 0;
+  if (!suspended)
     for (int s = tid; s < ns; s += NT) d.aExt[d.gAct[s]] = d.gExt[s];
   atomicAdd((unsigned long long*)&O[blockIdx.x].slot_visits, (unsigned long long)visits);
   atomicAdd((unsigned long long*)&O[blockIdx.x].cand_evals, (unsigned long long)evals);
   if (tid == 0) {
-    ls->rng = rng;
-    ls->tinc = tinc;
-    ls->vSind2 = vSind2;
-    ls->nGrain = nGrain;
-    ls->pool_used = pool_used;
-    ls->events = events;
-    ls->captures = captures;
-    ls->nucs = nucs;
-    ls->draws = draws;
+    const int status = go == -2 ? 2 : (fin == -1 ? 1 : (fin == -2 ? 2 : 0));
     ls->nslots = ns;
-    ls->done = suspended ? 0 : 1;
+    ls->done = status == 0 ? 1 : 0;
     amcd_out& o = O[blockIdx.x];
-    o.status = go == -1 ? 1 : (go == -2 ? 2 : 0);
-    o.n_events = events;
-    o.n_captures = captures;
-    o.n_nucleations = nucs;
+    o.status = status;
+    o.n_events = ls->events;
+    o.n_captures = ls->captures;
+    o.n_nucleations = ls->nucs;
     o.n_slots = ns;
-    o.nGrain = nGrain;
-    o.pool_used = pool_used;
-    o.draws = draws;
-    o.tinc = tinc;
+    o.nGrain = ls->nGrain;
+    o.pool_used = ls->pool_used;
+    o.tinc = ls->tinc;
   }
 }
 
@@ -1554,6 +1751,19 @@ static void collect_timing(amcd_batch* b) {
 
 int amcd_read_outs(amcd_batch* b, amcd_out* outs) { return read_outs(b, outs); }
 
+// capture-loop phase timers (zeros unless built with -DAMCD_PHASE_PROFILE); reading resets them
+int amcd_phase_profile(unsigned long long* out16) {
+#ifdef AMCD_PHASE_PROFILE
+  HIPT(hipDeviceSynchronize());
+  HIPT(hipMemcpyFromSymbol(out16, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * 16));
+  unsigned long long z[16] = {0};
+  HIPT(hipMemcpyToSymbol(HIP_SYMBOL(g_phase), z, sizeof(z)));
+#else
+  for (int q = 0; q < 16; ++q) out16[q] = 0;
+#endif
+  return 0;
+}
+
 int amcd_last_timings(amcd_batch* b, double* ms, int n) {
   for (int q = 0; q < n && q < 6; ++q) ms[q] = b->last_ms[q];
   return 0;
@@ -1591,12 +1801,13 @@ static int regrow_tier(amcd_batch* b, amcd_out* outs) {
     if (launch_event_loop(b) || read_outs(b, outs)) return -1;
   }
 }
-// smallest tier holding 1.5x the largest growing list of the step just done
+// smallest tier holding 1.125x the largest growing list of the step just done (a step
+// that outgrows it resumes in the next tier)
 static void pick_tier(amcd_batch* b, const amcd_out* outs) {
   int mx = 0;
   for (int i = 0; i < b->n; ++i) mx = std::max(mx, outs[i].n_slots);
   int t = 0;
-  while (t < 3 && kTierCap[t] < mx + mx / 2) ++t;
+  while (t < 3 && kTierCap[t] < mx + mx / 8) ++t;
   b->tier = t;
 }
 

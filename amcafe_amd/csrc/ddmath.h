@@ -11,7 +11,7 @@
 //
 // Arguments are restricted to what the capture geometry produces:
 //   amc_atan2_q1(y, x):  y >= 0, x >= 0, finite
-//   amc_sin_q1 / amc_cos_q1(x):  0 <= x <= pi/2
+//   amc_sin_q1 / amc_cos_q1(x):  0 <= x <= pi/2 (fast path when the result is >= 1/2)
 // Error-free transformations use explicit fma(); the file must be compiled with
 // -ffp-contract=off so the compiler does not fuse anything else.
 #pragma once
@@ -90,7 +90,7 @@ __host__ __device__ static inline amc_dd amc_atan_dd01(amc_dd q) {
 }
 
 // atan2(y, x) rounded to nearest, for y >= 0, x >= 0 (both finite).
-__host__ __device__ static inline double amc_atan2_q1(double y, double x) {
+__host__ __device__ static inline double amc_atan2_dd(double y, double x) {
   if (y == 0.0) return 0.0;
   if (x == 0.0) return kPi2[0][0];
   bool swap = y > x;
@@ -114,15 +114,113 @@ __host__ __device__ static inline void amc_sincos_dd(double x, amc_dd* s, amc_dd
   if (s) *s = amc_dd_add(amc_dd_mul(sk, cr), amc_dd_mul(ck, sr));
   if (c) *c = amc_dd_add(amc_dd_mul(ck, cr), amc_dd_neg(amc_dd_mul(sk, sr)));
 }
-__host__ __device__ static inline double amc_sin_q1(double x) {
+__host__ __device__ static inline double amc_sin_dd(double x) {
   amc_dd s;
   amc_sincos_dd(x, &s, nullptr);
   return s.hi + s.lo;
 }
-__host__ __device__ static inline double amc_cos_q1(double x) {
+__host__ __device__ static inline double amc_cos_dd(double x) {
   amc_dd c;
   amc_sincos_dd(x, nullptr, &c);
   return c.hi + c.lo;
+}
+
+// ---------------------------------------------------------------------------------
+// Fast paths with a rounding test (Ziv's strategy): an approximation hi + lo with
+// relative error below 2^-65 is rounded to hi when the exact value provably lies on
+// the same side of both neighbouring midpoints; otherwise the double-double routine
+// above decides.  The bound used in the test, 2^-62 relative, leaves a factor 8 of
+// margin; about 0.4% of calls take the slow path.
+__host__ __device__ static inline bool amc_round_ok(double hi, double lo) {  // hi > 0
+  const double err = hi * 0x1p-62;
+  const double up = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, hi) + 1ull) - hi;
+  const double dn = hi - __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, hi) - 1ull);
+  return (lo + err < 0.5 * up) && (lo - err > -0.5 * dn);
+}
+
+__host__ __device__ static inline double amc_atan2_q1(double y, double x) {
+  if (y == 0.0) return 0.0;
+  if (x == 0.0) return kPi2[0][0];
+  const bool swap = y > x;
+  const double num = swap ? x : y, den = swap ? y : x;
+  const double qh = num / den;
+  const double ql = fma(-qh, den, num) / den;  // num/den = qh + ql to 2^-106
+  const int i = (int)rint(qh * AMC_DD_TAB);
+  double rh, rl;
+  if (i == 0) {
+    rh = qh;
+    rl = ql;
+  } else {
+    const double c = (double)i / AMC_DD_TAB;
+    const double nh = qh - c;  // exact
+    amc_dd p = amc_two_prod(qh, c);
+    amc_dd s = amc_two_sum(1.0, p.hi);
+    const double dl = s.lo + p.lo + ql * c;
+    rh = nh / s.hi;
+    double e = fma(-rh, s.hi, nh);
+    e = e + ql - rh * dl;
+    rl = e / s.hi;
+  }
+  // atan(r) = r - r^3/3 + ... (|r| <= 1/128; the r^13 term is below 2^-84 r)
+  const double r2 = rh * rh;
+  const double P = -0x1.5555555555555p-2 +
+                   r2 * (0x1.999999999999ap-3 + r2 * (-0x1.2492492492492p-3 + r2 * (0x1.c71c71c71c71cp-4 +
+                                                                                        r2 * -0x1.745d1745d1746p-4)));
+  const double tail = rh * r2 * P - rl * r2;
+  amc_dd a = amc_two_sum(kAtanTab[i][0], rh);
+  double lo = a.lo + kAtanTab[i][1] + rl + tail;
+  amc_dd res = amc_fast_two_sum(a.hi, lo);
+  if (swap) {
+    amc_dd h = amc_two_sum(kPi2[0][0], -res.hi);
+    double l = h.lo + kPi2[0][1] - res.lo;
+    res = amc_fast_two_sum(h.hi, l);
+  }
+  if (amc_round_ok(res.hi, res.lo)) return res.hi;
+  return amc_atan2_dd(y, x);
+}
+
+// sin / cos for 0 <= x <= pi/2 whose result is at least 1/2 (cos on [0, pi/4], sin on
+// [pi/4, pi/2] in the capture test): x = k/64 + r, Taylor terms of r in double around
+// the exact leading products.
+__host__ __device__ static inline void amc_sincos_fast(double x, double* sh_out, double* sl_out, double* ch_out,
+                                                       double* cl_out) {
+  const int k = (int)rint(x * AMC_DD_TAB);
+  const double r = x - (double)k / AMC_DD_TAB;
+  const double r2 = r * r;
+  const double sm = r * r2 * (-0x1.5555555555555p-3 + r2 * (0x1.1111111111111p-7 + r2 * (-0x1.a01a01a01a01ap-13 +
+                                                                                        r2 * 0x1.71de3a556c734p-19)));
+  const double cm = r2 * (-0.5 + r2 * (0x1.5555555555555p-5 + r2 * (-0x1.6c16c16c16c17p-10 +
+                                                                    r2 * (0x1.a01a01a01a01ap-16 +
+                                                                          r2 * -0x1.27e4fb7789f5cp-22))));
+  const double sh = kSinTab[k][0], sl = kSinTab[k][1], ch = kCosTab[k][0], cl = kCosTab[k][1];
+  if (sh_out) {
+    amc_dd p = amc_two_prod(ch, r);
+    amc_dd s = amc_two_sum(sh, p.hi);
+    double lo = s.lo + p.lo + sl + cl * r + (sh * cm + ch * sm);
+    amc_dd res = amc_fast_two_sum(s.hi, lo);
+    *sh_out = res.hi;
+    *sl_out = res.lo;
+  }
+  if (ch_out) {
+    amc_dd p = amc_two_prod(-sh, r);
+    amc_dd s = amc_two_sum(ch, p.hi);
+    double lo = s.lo + p.lo + cl - sl * r + (ch * cm - sh * sm);
+    amc_dd res = amc_fast_two_sum(s.hi, lo);
+    *ch_out = res.hi;
+    *cl_out = res.lo;
+  }
+}
+__host__ __device__ static inline double amc_sin_q1(double x) {
+  double h, l;
+  amc_sincos_fast(x, &h, &l, nullptr, nullptr);
+  if (h >= 0.5 && amc_round_ok(h, l)) return h;
+  return amc_sin_dd(x);
+}
+__host__ __device__ static inline double amc_cos_q1(double x) {
+  double h, l;
+  amc_sincos_fast(x, nullptr, nullptr, &h, &l);
+  if (h >= 0.5 && amc_round_ok(h, l)) return h;
+  return amc_cos_dd(x);
 }
 
 // Shape factor of the decentred-octahedron capture test, S/VoxelsCA.C:415-418, 424:

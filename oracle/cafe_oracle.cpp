@@ -357,6 +357,9 @@ struct Grid {
 struct StepStats {
   long Na = 0, n_mushy0 = 0, iters = 0, captures = 0, nucleations = 0, draws = 0;
   double tinc = 0;
+  // event log of the step: captured voxel, capturing voxel (-1: nucleation), Dt
+  std::vector<int> ev_vox, ev_src;
+  std::vector<double> ev_dt;
 };
 
 struct Sim {
@@ -741,6 +744,9 @@ struct Sim {
         Dt = 0.0;
         ExtA[i1] = 0.0;
         s.nucleations++;
+        s.ev_vox.push_back(indA[i1]);
+        s.ev_src.push_back(-1);
+        s.ev_dt.push_back(Dt);
       } else {  // growth + capture + re-centring (S/VoxelsCA.C:506-607)
         for (int j = 0; j < (int)vhatvec.size(); ++j) {
           int i1 = vs2ind[j];
@@ -810,6 +816,9 @@ struct Sim {
         Cen[3 * (size_t)i1 + 1] = Cen[3 * (size_t)js + 1] + locX[1];
         Cen[3 * (size_t)i1 + 2] = Cen[3 * (size_t)js + 2] + locX[2];
         s.captures++;
+        s.ev_vox.push_back(indA[i1]);
+        s.ev_src.push_back(indA[js]);
+        s.ev_dt.push_back(Dt);
       }
       tinc += Dt;
     }
@@ -1033,6 +1042,17 @@ void orc_stats(void* h, long* o, double* tinc) {
   o[0] = s->st.Na; o[1] = s->st.n_mushy0; o[2] = s->st.iters; o[3] = s->st.captures;
   o[4] = s->st.nucleations; o[5] = s->st.draws;
   if (tinc) *tinc = s->st.tinc;
+}
+// event log of the last UpdateVoxels (grid indices); returns the number of events
+long orc_events(void* h, int* vox, int* src, double* dt, long cap) {
+  orc::Sim* s = (orc::Sim*)h;
+  long n = (long)s->st.ev_vox.size();
+  for (long q = 0; q < n && q < cap; ++q) {
+    vox[q] = s->st.ev_vox[q];
+    src[q] = s->st.ev_src[q];
+    dt[q] = s->st.ev_dt[q];
+  }
+  return n;
 }
 // standalone sampler and engine draws, for known-answer tests
 void orc_generate_samples(int n, unsigned int seed, double* out) {

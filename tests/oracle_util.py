@@ -33,6 +33,8 @@ def _load(path):
     L.orc_get_scalars.argtypes = [_P, _P]
     L.orc_set_laser.argtypes = [_P, _I, _I, _I]
     L.orc_stats.argtypes = [_P, _P, _P]
+    L.orc_events.argtypes = [_P, _P, _P, _P, ctypes.c_long]
+    L.orc_events.restype = ctypes.c_long
     L.orc_generate_samples.argtypes = [_I, ctypes.c_uint, _P]
     L.orc_uniform_stream.argtypes = [ctypes.c_ulong, _I, _P]
     L.orc_rot_mat.argtypes = [ctypes.c_double, _P, _P]
@@ -128,6 +130,14 @@ class Oracle:
         self.L.orc_stats(self.h, o.ctypes.data, t.ctypes.data)
         return dict(n_active=int(o[0]), n_mushy0=int(o[1]), n_events=int(o[2]), n_captures=int(o[3]),
                     n_nucleations=int(o[4]), draws=int(o[5]), tinc=float(t[0]))
+
+    def events(self):
+        """Event log of the last UpdateVoxels: captured voxel, capturing voxel (-1 for a
+        nucleation) and Dt, in event order (grid indices)."""
+        n = self.L.orc_events(self.h, None, None, None, 0)
+        vox, src, dt = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        self.L.orc_events(self.h, vox.ctypes.data, src.ctypes.data, dt.ctypes.data, n)
+        return vox, src, dt
 
     def done(self):
         return self.L.orc_done(self.h)
