@@ -59,11 +59,23 @@ def main():
     ap.add_argument("--workload", default="c1_raster", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (scale for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched form shards "
               "instances over GPUs with no communication, so N GPUs is N independent runs.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}), flush=True)
+        return
 
     import amcafe_amd
     deck = amcafe_amd.read_deck(os.path.join(ROOT, "decks", args.workload + ".in"))

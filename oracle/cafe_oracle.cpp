@@ -151,8 +151,12 @@ struct Orient {
     if (fabs(z) < 1e-10) {
       o[0] = 0; o[1] = 0; o[2] = 0;
     } else {
-      o[0] = pow(1 - (pow(x, 2) + pow(y, 2)) * M_PI / (24 * pow(z, 2)), 1.0 / 2.0) * x;
-      o[1] = pow(1 - (pow(x, 2) + pow(y, 2)) * M_PI / (24 * pow(z, 2)), 1.0 / 2.0) * y;
+      // GCC -O3 vectorises the reference's two identical pow(., 1/2) factors into one
+      // packed square root (the pow calls remain only for errno), so its result is
+      // the correctly rounded sqrt, which glibc's pow misses by an ulp now and then.
+      const double s = sqrt(1 - (pow(x, 2) + pow(y, 2)) * M_PI / (24 * pow(z, 2)));
+      o[0] = s * x;
+      o[1] = s * y;
       o[2] = pow(6 / M_PI, 1.0 / 2.0) * z - (pow(x, 2) + pow(y, 2)) * pow(M_PI, 1.0 / 2.0) / (pow(24, 1.0 / 2.0) * z);
     }
   }
