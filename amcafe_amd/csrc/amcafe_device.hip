@@ -788,7 +788,7 @@ struct NewCell {  // a growing cell created by the engine, taken over by its own
 };
 struct EngineShared {
   Best sb[2][16];   // per-wave bests, by event parity
-  double nextU[2];  // the uniform of the next event, by parity
+  double ubuf[2][64];  // uniforms of draws 64b .. 64b+63 of this launch in ubuf[b & 1]
   int susp[2];      // engine suspension request (-1 pool refill, -2 larger tier), by parity
   NewCell mb[2];    // cell created in the event of this parity
   int nslots;
@@ -797,15 +797,30 @@ struct EngineShared {
 };
 // The decision every wave takes after the barrier (S/VoxelsCA.C:476-485).
 struct Decision {
-  int stop, nuc, i1, cand, js;
+  int stop, nuc, i1, cand, js, wave;
   double dt;
 };
+// The table is read in full before any comparison, so the reads overlap; `nd` is the
+// number of draws this launch has consumed, which selects the event's uniform.
 template <int NW>
-__device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX) {
-  Best b = sh.sb[par][0];
+__device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX, int nd) {
+  Best e[NW];
+#pragma unroll
+  for (int q = 0; q < NW; ++q) e[q] = sh.sb[par][q];
+  const double U = sh.ubuf[(nd >> 6) & 1][nd & 63];
+  const int susp = sh.susp[par];
+  Best b = e[0];
+  int wq = 0;
+#pragma unroll
   for (int q = 1; q < NW; ++q) {
-    const Best o = sh.sb[par][q];
-    if (better(o, b)) b = o;
+    const bool bt = better(e[q], b);
+    b.t = bt ? e[q].t : b.t;
+    b.r = bt ? e[q].r : b.r;
+    b.key = bt ? e[q].key : b.key;
+    b.cand = bt ? e[q].cand : b.cand;
+    b.act = bt ? e[q].act : b.act;
+    b.tgt = bt ? e[q].tgt : b.tgt;
+    wq = bt ? q : wq;
   }
   Decision x;
   x.stop = 0;
@@ -813,18 +828,26 @@ __device__ static inline Decision decide(const EngineShared& sh, int par, double
   x.i1 = b.tgt;
   x.cand = b.cand;
   x.js = b.act;
+  x.wave = wq;
   x.dt = 0.0;
   if (!(T < DelT)) {
     x.stop = 1;  // loop condition (S/VoxelsCA.C:381)
-  } else if (sh.susp[par]) {
-    x.stop = sh.susp[par];
+  } else if (susp) {
+    x.stop = susp;
   } else if (b.cand < 0 || b.t >= 1e6) {
     x.stop = 1;  // S/VoxelsCA.C:480
   } else {
-    x.nuc = sh.nextU[par] < rX;
+    x.nuc = U < rX;
     x.dt = x.nuc ? 0.0 : b.t;
   }
   return x;
+}
+// Uniforms 64b .. 64b+63 of a launch whose engine state was x0 at its start: lane j
+// jumps to x0 * 16807^(128b + 2j + 1) (aj = 16807^(2j+1), cb = 16807^(128b)).
+__device__ static inline double draw_batch(unsigned int x0, unsigned int cb, unsigned int aj) {
+  const unsigned int xb = mod_m((unsigned long long)x0 * cb);
+  const unsigned int x1 = mod_m((unsigned long long)xb * aj);
+  return canonical(x1, lcg_next(x1));
 }
 
 template <int NT, int K>
@@ -859,6 +882,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, doubl
   }
   int skip = -1;  // voxel the engine may be capturing right now
   int par = 0;
+  int nd = 0;     // draws consumed by this launch
   int nev = 0;    // events since the bounds were last recomputed
   Decision x;
   x.stop = 0;
@@ -932,8 +956,9 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, doubl
       if (lane == 0) sh.sb[par][w] = b;
       __syncthreads();
       // ---- DECIDE
-      x = decide<NW>(sh, par, T, p.DelT, p.rX);
+      x = decide<NW>(sh, par, T, p.DelT, p.rX, nd);
       if (x.stop) break;
+      ++nd;
     }
     apply_first = false;
     // ---- APPLY
@@ -1046,13 +1071,15 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     int vSind2 = ls->vSind2, nGrain = ls->nGrain, pool_used = ls->pool_used;
     int events = ls->events, captures = ls->captures, nucs = ls->nucs;
     int nslots = ls->nslots;
-    // lane 1 draws each uniform one event ahead
-    unsigned int rng = ls->rng;
-    long long draws = ls->draws;
-    if (go > 0 && lane == 1) {
-      unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
-      sh.nextU[go == 2 ? 1 : 0] = canonical(x1, x2);
-      rng = x2;
+    // the uniforms, 64 per batch, drawn ahead by jump-ahead (S/VoxelsCA.C:483)
+    const unsigned int x0 = ls->rng;
+    const long long draws0 = ls->draws;
+    const unsigned int aj = lcg_pow(2ull * lane + 1), c128 = lcg_pow(128);
+    unsigned int cb = 1;  // 16807^(128 b) for the next batch b to write
+    int nd = 0;
+    if (go > 0) {
+      sh.ubuf[0][lane] = draw_batch(x0, cb, aj);
+      cb = mod_m((unsigned long long)cb * c128);
     }
     int par = 0, fin = 0;
     Decision x;
@@ -1069,18 +1096,17 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         PHASE(et, 0, tid == 0);
         __syncthreads();
         PHASE(et, 1, tid == 0);
-        x = decide<NW>(sh, par, tinc, p.DelT, p.rX);
+        x = decide<NW>(sh, par, tinc, p.DelT, p.rX, nd);
         if (x.stop) {
           fin = x.stop < 0 ? x.stop : 0;
           break;
         }
         events++;
-        if (lane == 1) {  // the draw was consumed: prepare the next one
-          draws += 2;
-          unsigned int x1 = lcg_next(rng), x2 = lcg_next(x1);
-          sh.nextU[par ^ 1] = canonical(x1, x2);
-          rng = x2;
+        if ((nd & 63) == 0) {  // first draw of batch nd/64: write batch nd/64 + 1
+          sh.ubuf[((nd >> 6) + 1) & 1][lane] = draw_batch(x0, cb, aj);
+          cb = mod_m((unsigned long long)cb * c128);
         }
+        ++nd;
         PHASE(et, 2, tid == 0);
       }
       apply_first = false;
@@ -1097,21 +1123,30 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         }
         susp = -1;
       } else {
-        // loads that depend only on i1 go first, so they arrive during the re-centring
-        const int tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
-        const int v1 = d.aVox[i1];
-        const int cold = d.aCold[i1];
-        const double T1 = d.aT[i1];
-        const double rq1 = d.ruq[i1];
-        int vst = 0, vt = 0;
-        double rqt = 0.0;
+        int tn, v1, cold, vst = 0, vt = 0, G = 1, vjs = 0;
+        double T1, rq1, rqt = 0.0, fcap = 0.0;
+        double R[9], c0[3], cen[3], E0;
+        tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
+        v1 = d.aVox[i1];
+        cold = d.aCold[i1];
+        T1 = d.aT[i1];
+        rq1 = d.ruq[i1];
         if (tn >= 0) {
           vst = d.aVS[tn];
           vt = d.aVox[tn];
           rqt = d.ruq[tn];
         }
-        double R[9], cen[3], E0;
-        int G;
+        if (!x.nuc) {
+          const int js = x.js;
+          G = min(max(d.aG[js], 1), d.gcap);
+          const double* Rg = d.rot + 9 * (size_t)(G - 1);
+          for (int q = 0; q < 9; ++q) R[q] = Rg[q];
+          c0[0] = d.aCx[js];
+          c0[1] = d.aCy[js];
+          c0[2] = d.aCz[js];
+          vjs = d.aVox[js];
+          fcap = d.cF[x.cand];
+        }
         if (x.nuc) {  // S/VoxelsCA.C:486-506
           const double* P = d.pool + 9 * (size_t)pool_used;
           for (int q = 0; q < 9; ++q) R[q] = P[q];
@@ -1124,12 +1159,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           nucs++;
           if (lane == 0) ls->pending = 0;
         } else {      // S/VoxelsCA.C:507-605
-          const int js = x.js;
-          G = min(max(d.aG[js], 1), d.gcap);
-          const double* Rg = d.rot + 9 * (size_t)(G - 1);
-          for (int q = 0; q < 9; ++q) R[q] = Rg[q];
-          const double c0[3] = {d.aCx[js], d.aCy[js], d.aCz[js]};
-          recentre(d, d.aVox[js], v1, c0, R, d.cF[x.cand], cen, &E0);
+          recentre(d, vjs, v1, c0, R, fcap, cen, &E0);
           captures++;
         }
         const int scan = vSind2;
@@ -1226,11 +1256,10 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
       ls->captures = captures;
       ls->nucs = nucs;
     }
-    if (tid == 1 && go != -2) {
-      // lane 1 holds the engine one uniform ahead: store the state before that draw
-      ls->rng = go > 0 ? rng_back(rng) : rng;
-      ls->draws = draws;
-      O[blockIdx.x].draws = draws;
+    if (tid == 1 && go != -2) {  // engine state after the nd draws of this launch
+      ls->rng = mod_m((unsigned long long)x0 * lcg_pow(2ull * nd));
+      ls->draws = draws0 + 2ll * nd;
+      O[blockIdx.x].draws = draws0 + 2ll * nd;
     }
   }
   __syncthreads();

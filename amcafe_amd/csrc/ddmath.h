@@ -138,14 +138,15 @@ __host__ __device__ static inline bool amc_round_ok(double hi, double lo) {  // 
   return (lo + err < 0.5 * up) && (lo - err > -0.5 * dn);
 }
 
-__host__ __device__ static inline double amc_atan2_q1(double y, double x) {
-  if (y == 0.0) return 0.0;
-  if (x == 0.0) return kPi2[0][0];
+// Fast part of atan2(y, x) for y, x >= 0 finite: hi + lo within 2^-65 relative of the
+// exact value; no rounding test and no special cases (the callers select those).
+__host__ __device__ static inline void amc_atan2_fast(double y, double x, double* hi_out, double* lo_out) {
   const bool swap = y > x;
   const double num = swap ? x : y, den = swap ? y : x;
   const double qh = num / den;
   const double ql = fma(-qh, den, num) / den;  // num/den = qh + ql to 2^-106
-  const int i = (int)rint(qh * AMC_DD_TAB);
+  // 0/0 (a special case the callers replace) must still give an in-range table index
+  const int i = (int)rint(((qh >= 0.0) & (qh <= 1.0) ? qh : 0.0) * AMC_DD_TAB);
   double rh, rl;
   if (i == 0) {
     rh = qh;
@@ -175,7 +176,15 @@ __host__ __device__ static inline double amc_atan2_q1(double y, double x) {
     double l = h.lo + kPi2[0][1] - res.lo;
     res = amc_fast_two_sum(h.hi, l);
   }
-  if (amc_round_ok(res.hi, res.lo)) return res.hi;
+  *hi_out = res.hi;
+  *lo_out = res.lo;
+}
+__host__ __device__ static inline double amc_atan2_q1(double y, double x) {
+  if (y == 0.0) return 0.0;
+  if (x == 0.0) return kPi2[0][0];
+  double hi, lo;
+  amc_atan2_fast(y, x, &hi, &lo);
+  if (amc_round_ok(hi, lo)) return hi;
   return amc_atan2_dd(y, x);
 }
 
@@ -184,7 +193,8 @@ __host__ __device__ static inline double amc_atan2_q1(double y, double x) {
 // the exact leading products.
 __host__ __device__ static inline void amc_sincos_fast(double x, double* sh_out, double* sl_out, double* ch_out,
                                                        double* cl_out) {
-  const int k = (int)rint(x * AMC_DD_TAB);
+  // arguments outside [0, pi/2] (from special cases the callers replace) still index the table
+  const int k = (int)rint(((x >= 0.0) & (x <= 1.58) ? x : 0.0) * AMC_DD_TAB);  // k <= 101
   const double r = x - (double)k / AMC_DD_TAB;
   const double r2 = r * r;
   const double sm = r * r2 * (-0x1.5555555555555p-3 + r2 * (0x1.1111111111111p-7 + r2 * (-0x1.a01a01a01a01ap-13 +
@@ -227,7 +237,10 @@ __host__ __device__ static inline double amc_cos_q1(double x) {
 //   pow(cos(th) * sin(ph), .5) with th, ph from the centre-to-centroid vector.
 // The operation order (products, sum, folds) follows the reference exactly; the
 // libm calls are the correctly rounded versions above.
-__host__ __device__ static inline double amc_shape_factor(double dx, double dy, double dz) {
+// All four fast parts run as one branch-free sequence (the two atan2 chains are
+// independent and overlap); only when some rounding test fails, or a special case
+// applies, is the whole value recomputed call by call with the slow paths.
+__host__ __device__ static inline double amc_shape_factor_ref(double dx, double dy, double dz) {
   const double pi4 = 0x1.921fb54442d18p-1, pi2 = 0x1.921fb54442d18p+0;  // M_PI/4.0, M_PI/2.0
   double th = amc_atan2_q1(fabs(dy), fabs(dx));
   th > pi4 ? th = pi2 - th : th;
@@ -235,4 +248,22 @@ __host__ __device__ static inline double amc_shape_factor(double dx, double dy, 
   double ph = amc_atan2_q1(rho, fabs(dz));
   ph < pi4 ? ph = pi2 - ph : ph;
   return sqrt(amc_cos_q1(th) * amc_sin_q1(ph));
+}
+__host__ __device__ static inline double amc_shape_factor(double dx, double dy, double dz) {
+  const double pi4 = 0x1.921fb54442d18p-1, pi2 = 0x1.921fb54442d18p+0;  // M_PI/4.0, M_PI/2.0
+  const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
+  const double rho = sqrt(dx * dx + dy * dy);
+  double t1h, t1l, t2h, t2l;
+  amc_atan2_fast(ay, ax, &t1h, &t1l);
+  amc_atan2_fast(rho, az, &t2h, &t2l);
+  double th = t1h, ph = t2h;
+  th > pi4 ? th = pi2 - th : th;
+  ph < pi4 ? ph = pi2 - ph : ph;
+  double ch, cl, sh, sl;
+  amc_sincos_fast(th, nullptr, nullptr, &ch, &cl);
+  amc_sincos_fast(ph, &sh, &sl, nullptr, nullptr);
+  const bool ok = (ay > 0.0) & (ax > 0.0) & (rho > 0.0) & (az > 0.0) & amc_round_ok(t1h, t1l) &
+                  amc_round_ok(t2h, t2l) & (ch >= 0.5) & amc_round_ok(ch, cl) & (sh >= 0.5) & amc_round_ok(sh, sl);
+  if (ok) return sqrt(ch * sh);
+  return amc_shape_factor_ref(dx, dy, dz);
 }
