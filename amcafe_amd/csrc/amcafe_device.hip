@@ -66,6 +66,7 @@ struct LoopState {
   int pending;       // 1: a nucleation was decided but the pool was empty
   int pend_tgt;      // active index of the voxel to nucleate
   int Na, nmushy0;
+  int zhot0, zhot1;  // z planes the last temperature evaluation could have melted
 };
 
 struct Inst {
@@ -79,6 +80,7 @@ struct Inst {
   double* T;
   uint8_t* bj;
   int* amap;
+  unsigned* zliq;  // per z plane: bit 0 = had liquid at the last active-set pass (or unknown)
   int *bcA, *bcM;  // per-ZB-block counts -> exclusive offsets
   // active list
   int acap;
@@ -87,14 +89,17 @@ struct Inst {
   int* aG;
   double *aT, *aExt, *aCx, *aCy, *aCz, *ruq;
   int* aNbr;
+  double* aRN;     // ruqs .5*(ruq[a] + ruq[aNbr[a][q]]) of every active cell a and neighbour q
   uint8_t* aCold;
   int* aScan;
   // growing list + candidates (26 slots per growing cell)
   int *gAct, *gScan;
   double *gExt, *gVhat;
+  double *gR, *gC;  // per slot: the cell's grain rotation (9) and centroid (3)
+  int *gV, *gG;     // per slot: the cell's voxel and grain
   uint8_t *gN, *gDead, *gGrow;
   double *cD, *cF, *cR;
-  int* cTgt;
+  int *cTgt, *cV;   // per entry: target (active index) and its voxel
   uint8_t* cK;
   // grains
   int gcap;
@@ -167,6 +172,11 @@ __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, 
   if (st.skip) return;
   int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
   bool hot = false;
+  if (v == 0) {  // planes with (k+.5)dz in (lz - max depth, lz] are the only ones T can reach tL in
+    const double depth = fmax(d.p.a1[2], d.p.a1[5]);
+    d.ls->zhot0 = max(0, (int)floor((st.lz - depth) / d.p.dz - 0.5) - 1);
+    d.ls->zhot1 = min(d.p.nz - 1, (int)floor(st.lz / d.p.dz - 0.5) + 1);
+  }
   if (v < d.N) {
     int i, j, k;
     decode(d, v, i, j, k);
@@ -237,11 +247,20 @@ __global__ __launch_bounds__(GB) void k_solid_to_mushy(Inst* I, const int* bstar
   if (S[ii].skip) return;
   int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
   if (v >= d.N) return;
-  uint8_t s = d.vs[v];
-  if ((s & 3) != 3) return;
   int i, j, k;
   decode(d, v, i, j, k);
   const int nx = d.p.nx, ny = d.p.ny, nz = d.p.nz;
+  // liquid exists only in planes that held it at the last active-set pass or that the
+  // last temperature evaluation could melt; other planes' solid voxels cannot change
+  const int h0 = d.ls->zhot0, h1 = d.ls->zhot1;
+  bool may = false;
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = k + dz;
+    if (z >= 0 && z < nz) may = may || (d.zliq[z] & 1u) || (z >= h0 && z <= h1);
+  }
+  if (!may) return;
+  uint8_t s = d.vs[v];
+  if ((s & 3) != 3) return;
   for (int dz = -1; dz <= 1; ++dz) {
     int z = k + dz;
     if (z < 0 || z >= nz) continue;
@@ -261,6 +280,12 @@ __global__ __launch_bounds__(GB) void k_solid_to_mushy(Inst* I, const int* bstar
   }
 }
 
+// The per-plane liquid flags are rebuilt by every active-set pass.
+__global__ void k_zliq_clear(Inst* I, const amcd_step* S) {
+  const Inst& d = I[blockIdx.x];
+  if (S[blockIdx.x].skip) return;
+  for (int z = threadIdx.x; z < d.p.nz; z += blockDim.x) d.zliq[z] = 0u;
+}
 // Active set: {mushy} U {liquid below tL} U {neighbours at or above tL of those}.
 // Writes bit 6 of vs and the per-block (ZB voxels) counts of active and mushy cells.
 __global__ __launch_bounds__(ZB) void k_active_flags(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
@@ -302,6 +327,16 @@ __global__ __launch_bounds__(ZB) void k_active_flags(Inst* I, const int* bstart,
     }
     d.vs[v] = (s & ~0x40) | (act ? 0x40 : 0);
   }
+  // planes holding liquid now (for the next step's solid->mushy pass): one atomic per run
+  // of liquid lanes in a plane
+  {
+    const bool liq = v < d.N && (d.vs[v] & 3) == 1;
+    const int kz = v < d.N ? v / d.nxy : 0;
+    const int lane = threadIdx.x & 63;
+    const int kprev = __shfl_up(kz, 1);
+    const bool lprev = __shfl_up((int)liq, 1) != 0;
+    if (liq && (lane == 0 || !lprev || kprev != kz)) atomicOr(&d.zliq[kz], 1u);
+  }
   unsigned long long ba = __ballot(act), bm = __ballot(mushy);
   int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
@@ -328,9 +363,20 @@ __global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd
   int per = (nb + 1023) / 1024;
   int b0 = t * per, b1 = min(nb, b0 + per);
   int a = 0, m = 0;
-  for (int b = b0; b < b1; ++b) {
-    a += d.bcA[b];
-    m += d.bcM[b];
+  // each thread's run of block counts is read 16 at a time with independent loads, so a
+  // chunk costs one memory latency instead of sixteen
+  for (int b = b0; b < b1; b += 16) {
+    int va[16], vm[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      va[q] = (b + q < b1) ? d.bcA[b + q] : 0;
+      vm[q] = (b + q < b1) ? d.bcM[b + q] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      a += va[q];
+      m += vm[q];
+    }
   }
   sA[t] = a;
   sM[t] = m;
@@ -343,12 +389,22 @@ __global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd
     __syncthreads();
   }
   int ra = sA[t] - a, rm = sM[t] - m;
-  for (int b = b0; b < b1; ++b) {
-    int ca = d.bcA[b], cm = d.bcM[b];
-    d.bcA[b] = ra;
-    d.bcM[b] = rm;
-    ra += ca;
-    rm += cm;
+  for (int b = b0; b < b1; b += 16) {
+    int va[16], vm[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      va[q] = (b + q < b1) ? d.bcA[b + q] : 0;
+      vm[q] = (b + q < b1) ? d.bcM[b + q] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      if (b + q < b1) {
+        d.bcA[b + q] = ra;
+        d.bcM[b + q] = rm;
+      }
+      ra += va[q];
+      rm += vm[q];
+    }
   }
   if (t == 1023) {
     int Na = sA[1023], Nm = sM[1023];
@@ -436,6 +492,15 @@ __device__ static inline void centre(const Inst& d, int v, double* c) {
 
 // Capture candidate of growing cell a (centroid c, rotation R) towards liquid voxel
 // vt: D = pow(cos th sin ph,.5) * |R^T dnx|_1 and the factor f (S/VoxelsCA.C:409-424).
+__device__ static inline void candidate_at(const double* ct, const double* c, const double* R, double* D, double* f) {
+  double dnx0 = ct[0] - c[0], dnx1 = ct[1] - c[1], dnx2 = ct[2] - c[2];
+  double ff = amc_shape_factor(dnx0, dnx1, dnx2);
+  double d0 = fabs(R[0] * dnx0 + R[3] * dnx1 + R[6] * dnx2);
+  double d1 = fabs(R[1] * dnx0 + R[4] * dnx1 + R[7] * dnx2);
+  double d2 = fabs(R[2] * dnx0 + R[5] * dnx1 + R[8] * dnx2);
+  *D = ff * (d0 + d1 + d2);
+  *f = ff;
+}
 __device__ static inline void candidate(const Inst& d, const double* c, const double* R, int vt, double* D,
                                         double* f) {
   double ct[3];
@@ -485,6 +550,7 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
           int n = -1;
           if (x >= 0 && x < d.p.nx && y >= 0 && y < d.p.ny && z >= 0 && z < d.p.nz)
             n = d.amap[v + dz * d.nxy + dy * d.p.nx + dx];
+          d.aRN[(size_t)a * AMCD_NB + q] = n >= 0 ? .5 * (d.ruq[a] + d.ruq[n]) : 0.0;
           nb[q++] = n;
           if (n >= 0) {
             if (!(d.aT[n] < d.p.tL)) cold = false;
@@ -521,7 +587,13 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
   }
   const double* R = d.rot + 9 * (size_t)(G - 1);
   double Rl[9];
-  for (int q = 0; q < 9; ++q) Rl[q] = R[q];
+  for (int q = 0; q < 9; ++q) {
+    Rl[q] = R[q];
+    d.gR[9 * (size_t)slot + q] = R[q];
+  }
+  for (int q = 0; q < 3; ++q) d.gC[3 * (size_t)slot + q] = c[q];
+  d.gV[slot] = d.aVox[a];
+  d.gG[slot] = G;
   const int* nb = d.aNbr + (size_t)a * AMCD_NB;
   int n = 0;
   size_t cb = (size_t)slot * AMCD_NB;
@@ -534,6 +606,7 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
     d.cF[cb + n] = f;
     d.cR[cb + n] = .5 * (d.ruq[a] + d.ruq[t]);
     d.cTgt[cb + n] = t;
+    d.cV[cb + n] = d.aVox[t];
     d.cK[cb + n] = (uint8_t)q;
     ++n;
   }
@@ -541,7 +614,7 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
   const int kb = d.aScan[a] * 32;
   for (int x = 1; x < n; ++x) {
     double D = d.cD[cb + x], f = d.cF[cb + x], r = d.cR[cb + x];
-    int t = d.cTgt[cb + x];
+    int t = d.cTgt[cb + x], tv = d.cV[cb + x];
     uint8_t kk = d.cK[cb + x];
     int y = x - 1;
     while (y >= 0 && cand_less(D, r, kb + kk, d.cD[cb + y], d.cR[cb + y], kb + d.cK[cb + y])) {
@@ -549,6 +622,7 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
       d.cF[cb + y + 1] = d.cF[cb + y];
       d.cR[cb + y + 1] = d.cR[cb + y];
       d.cTgt[cb + y + 1] = d.cTgt[cb + y];
+      d.cV[cb + y + 1] = d.cV[cb + y];
       d.cK[cb + y + 1] = d.cK[cb + y];
       --y;
     }
@@ -556,6 +630,7 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
     d.cF[cb + y + 1] = f;
     d.cR[cb + y + 1] = r;
     d.cTgt[cb + y + 1] = t;
+    d.cV[cb + y + 1] = tv;
     d.cK[cb + y + 1] = kk;
   }
   d.gN[slot] = (uint8_t)n;
@@ -732,24 +807,54 @@ __device__ static inline bool cand_less(double Da, double Ra, int ka, double Db,
   return (c1 & nb) | (!c1 & c2 & (Da < Db)) | (!c1 & !c2 & tail);
 }
 
-// First live entry at or after position `from` of slot s's sorted list.  Entries
-// before the first live one stay dead: a captured voxel never becomes liquid again
-// within the step.  `skip` is a voxel that counts as dead although its state may not
-// be written yet.
+// Liveness of candidate targets during the loop: an LDS bitmap over active cells with
+// bit set = "not liquid" (filled from the state at launch, then set by the engine for
+// every voxel it captures or nucleates).  Every list entry was liquid when the lists
+// were built, so a clear bit means live.
+#define AMCD_DEAD_BITS (1 << 19)
+__device__ static inline bool dead_bit(const unsigned* dead, int t) { return (dead[t >> 5] >> (t & 31)) & 1u; }
+__device__ static inline bool live_target(const unsigned* dead, int t, int skip) {
+  return t >= 0 && t != skip && !dead_bit(dead, t);
+}
+
+// First live entry at or after position `from` of slot s's sorted list, read four
+// entries at a time (one round of loads in the common case).  Entries before the
+// first live one stay dead: a captured voxel never becomes liquid again within the
+// step.  `skip` is a voxel that counts as dead although its bit may not be set yet.
 struct SlotHead {
   double D1;
   int c1, tgt1;  // tgt1 = -1: no live entry
 };
-__device__ static inline bool live_target(const Inst& d, int t, int skip) { return t != skip && d.aVS[t] == 1; }
-__device__ static inline SlotHead slot_head(const Inst& d, int s, int from, int skip) {
+__device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead, int s, int from, int skip) {
   SlotHead h;
   const int n = d.gN[s];
   const size_t cb = (size_t)s * AMCD_NB;
-  int p = from;
-  while (p < n && !live_target(d, d.cTgt[cb + p], skip)) ++p;
-  h.c1 = (int)cb + p;
-  h.tgt1 = (p < n) ? d.cTgt[cb + p] : -1;
-  h.D1 = (p < n) ? d.cD[cb + p] : __builtin_inf();
+  h.c1 = (int)cb + n;
+  h.tgt1 = -1;
+  h.D1 = __builtin_inf();
+  for (int p = from; p < n; p += 4) {
+    int t[4];
+    double D[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      t[q] = (p + q < n) ? d.cTgt[cb + p + q] : -1;
+      D[q] = (p + q < n) ? d.cD[cb + p + q] : 0.0;
+    }
+    int hit = -1;
+#pragma unroll
+    for (int q = 3; q >= 0; --q)
+      if (live_target(dead, t[q], skip)) hit = q;
+    if (hit >= 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q == hit) {
+          h.c1 = (int)cb + p + q;
+          h.tgt1 = t[q];
+          h.D1 = D[q];
+        }
+      break;
+    }
+  }
   return h;
 }
 
@@ -851,8 +956,8 @@ __device__ static inline double draw_batch(unsigned int x0, unsigned int cb, uns
 }
 
 template <int NT, int K>
-__device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, double T, long long& visits,
-                                  long long& evals) {
+__device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
+                                  long long& visits, long long& evals) {
   constexpr int NO = NT - 64;  // owner threads
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -874,7 +979,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, doubl
     if (loaded && s < ns0) {
       E[k] = d.gExt[s];
       V[k] = d.gVhat[s];
-      const SlotHead h = slot_head(d, s, 0, -1);
+      const SlotHead h = slot_head(d, dead, s, 0, -1);
       c1[k] = h.c1;
       tgt1[k] = h.tgt1;
       if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
@@ -934,7 +1039,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, doubl
               const int n = d.gN[s];
               for (int q = pos + 1; q < n; ++q) {
                 const int tq = d.cTgt[c - pos + q];
-                if (!live_target(d, tq, skip)) continue;
+                if (!live_target(dead, tq, skip)) continue;
                 evals++;
                 const double tt = (d.cD[c - pos + q] - e) / v;
                 if (!(tt == t1)) break;
@@ -1007,7 +1112,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, int go, doubl
       if (!hit) continue;
       const int s = mo + k * NO;
       if (tgt1[k] == i1) {
-        const SlotHead h = slot_head(d, s, c1[k] - s * AMCD_NB + 1, i1);
+        const SlotHead h = slot_head(d, dead, s, c1[k] - s * AMCD_NB + 1, i1);
         c1[k] = h.c1;
         tgt1[k] = h.tgt1;
         lo[k] = hi[k] = fInf;
@@ -1037,12 +1142,24 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   constexpr int CAP = (NT - 64) * K;
   __shared__ EngineShared sh;
   __shared__ int s_go;
+  __shared__ unsigned s_dead[AMCD_DEAD_BITS / 32];
   const Inst& d = I[blockIdx.x];
   LoopState* ls = d.ls;
   if (ls->done) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const amcd_params& p = d.p;
   const amcd_step& st = S[blockIdx.x];
+  const int Na = ls->Na;
+  if (Na <= AMCD_DEAD_BITS) {  // liveness bitmap from the current states
+    for (int wd = tid; wd < (Na + 31) / 32; wd += NT) {
+      unsigned m = 0;
+      for (int q = 0; q < 32; ++q) {
+        const int a = 32 * wd + q;
+        if (a >= Na || d.aVS[a] != 1) m |= 1u << q;
+      }
+      s_dead[wd] = m;
+    }
+  }
   if (tid == 0) {
     sh.nslots = ls->nslots;
     sh.mb[0].valid = sh.mb[1].valid = 0;
@@ -1051,7 +1168,9 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     sh.fin = 0;
     sh.r_i1 = -1;
     s_go = 1;
-    if (ls->nslots > CAP) {
+    if (Na > AMCD_DEAD_BITS) {
+      s_go = -3;  // more interface cells than the liveness bitmap holds
+    } else if (ls->nslots > CAP) {
       s_go = -2;  // more growing cells than this tier holds
     } else if (ls->pending) {
       sh.r_i1 = ls->pend_tgt;
@@ -1064,7 +1183,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const int go = s_go;
   long long visits = 0, evals = 0;
   if (w > 0) {
-    owner_loop<NT, K>(d, sh, go, ls->tinc, visits, evals);
+    owner_loop<NT, K>(d, sh, s_dead, go, ls->tinc, visits, evals);
   } else {
     // engine state, identical in every lane of wave 0
     double tinc = ls->tinc;
@@ -1088,6 +1207,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     x.i1 = sh.r_i1;
     x.cand = -1;
     x.js = -1;
+    x.wave = 0;
     x.dt = 0.0;
     bool apply_first = go == 2;
     PHASE_INIT(et);
@@ -1123,29 +1243,25 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         }
         susp = -1;
       } else {
-        int tn, v1, cold, vst = 0, vt = 0, G = 1, vjs = 0;
-        double T1, rq1, rqt = 0.0, fcap = 0.0;
-        double R[9], c0[3], cen[3], E0;
-        tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
-        v1 = d.aVox[i1];
-        cold = d.aCold[i1];
-        T1 = d.aT[i1];
-        rq1 = d.ruq[i1];
-        if (tn >= 0) {
-          vst = d.aVS[tn];
-          vt = d.aVox[tn];
-          rqt = d.ruq[tn];
-        }
+        // everything the event needs, in one round of independent loads: the captured
+        // voxel's neighbour table and pair ruqs, and the winning entry's capturing cell
+        // (per-slot rotation, centroid, voxel, grain) and shape factor
+        const int cand = x.cand, sw = cand >= 0 ? cand / AMCD_NB : 0;
+        const int tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
+        const double rpair = (lane < AMCD_NB) ? d.aRN[(size_t)i1 * AMCD_NB + lane] : 0.0;
+        const int cold = d.aCold[i1];
+        const double T1 = d.aT[i1];
+        const int v1 = cand >= 0 ? d.cV[cand] : d.aVox[i1];  // (no entry: a resumed nucleation)
+        int vjs = 0, G = 1;
+        double fcap = 0.0, R[9], c0[3], cen[3], E0;
         if (!x.nuc) {
-          const int js = x.js;
-          G = min(max(d.aG[js], 1), d.gcap);
-          const double* Rg = d.rot + 9 * (size_t)(G - 1);
-          for (int q = 0; q < 9; ++q) R[q] = Rg[q];
-          c0[0] = d.aCx[js];
-          c0[1] = d.aCy[js];
-          c0[2] = d.aCz[js];
-          vjs = d.aVox[js];
-          fcap = d.cF[x.cand];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)sw + q];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) c0[q] = d.gC[3 * (size_t)sw + q];
+          fcap = d.cF[cand];
+          vjs = d.gV[sw];
+          G = d.gG[sw];
         }
         if (x.nuc) {  // S/VoxelsCA.C:486-506
           const double* P = d.pool + 9 * (size_t)pool_used;
@@ -1171,12 +1287,13 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           d.aExt[i1] = E0;
           d.aVS[i1] = 2;
           d.aScan[i1] = scan;
+          s_dead[i1 >> 5] |= 1u << (i1 & 31);
         }
         vSind2++;
         tinc += x.dt;
         PHASE(et, 3, tid == 0);
         // the new mushy voxel joins the growing list if it has liquid neighbours
-        const bool liq = tn >= 0 && vst == 1;
+        const bool liq = live_target(s_dead, tn, i1);
         const unsigned long long bal = __ballot(liq);
         if (cold && bal) {
           const int slot = nslots;
@@ -1185,9 +1302,17 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           const double V0 = growth_velocity(p, T1, &err);
           double D = __builtin_inf(), f = 0, r = __builtin_inf();
           int key = 0x7fffffff;
+          // neighbour lane of v1, in the order of the active-neighbour table
+          int vi, vj, vk;
+          decode(d, v1, vi, vj, vk);
+          const int qq = lane < 13 ? lane : lane + 1;
+          const int ddx = qq % 3 - 1, ddy = (qq / 3) % 3 - 1, ddz = qq / 9 - 1;
+          const int vt = v1 + ddz * d.nxy + ddy * p.nx + ddx;
           if (liq) {
-            candidate(d, cen, R, vt, &D, &f);
-            r = .5 * (rq1 + rqt);
+            const double ct[3] = {((double)(vi + ddx) + .5) * p.dx, ((double)(vj + ddy) + .5) * p.dy,
+                                  ((double)(vk + ddz) + .5) * p.dz};
+            candidate_at(ct, cen, R, &D, &f);
+            r = rpair;
             key = scan * 32 + lane;
           }
           int rank = 0;  // position in (D, ruqs, key) order
@@ -1201,6 +1326,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             d.cF[cb + rank] = f;
             d.cR[cb + rank] = r;
             d.cTgt[cb + rank] = tn;
+            d.cV[cb + rank] = vt;
             d.cK[cb + rank] = (uint8_t)lane;
             if (rank == 0) {
               nc.D1 = D;
@@ -1221,7 +1347,19 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           nb = wave_best(nb);
           visits++;
           evals += __popcll(bal);
+          // the slot's static capture inputs (rotation, centroid, voxel)
+          double rv = 0.0;
+#pragma unroll
+          for (int q = 0; q < 9; ++q)
+            if (q == lane) rv = R[q];
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            if (q + 9 == lane) rv = cen[q];
+          if (lane < 9) d.gR[9 * (size_t)slot + lane] = rv;
+          if (lane >= 9 && lane < 12) d.gC[3 * (size_t)slot + lane - 9] = rv;
           if (lane == 0) {
+            d.gV[slot] = v1;
+            d.gG[slot] = G;
             d.gAct[slot] = i1;
             d.gScan[slot] = scan;
             d.gExt[slot] = E0;
@@ -1266,13 +1404,13 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   // ---- loop state and counters
   const int ns = sh.nslots;
   const int fin = sh.fin;  // 0 finished, -1 pool refill, -2 larger tier
-  const bool suspended = go == -2 || fin < 0;
+  const bool suspended = go == -2 || go == -3 || fin < 0;
   if (!suspended)
     for (int s = tid; s < ns; s += NT) d.aExt[d.gAct[s]] = d.gExt[s];
   atomicAdd((unsigned long long*)&O[blockIdx.x].slot_visits, (unsigned long long)visits);
   atomicAdd((unsigned long long*)&O[blockIdx.x].cand_evals, (unsigned long long)evals);
   if (tid == 0) {
-    const int status = go == -2 ? 2 : (fin == -1 ? 1 : (fin == -2 ? 2 : 0));
+    const int status = go == -3 ? -3 : (go == -2 ? 2 : (fin == -1 ? 1 : (fin == -2 ? 2 : 0)));
     ls->nslots = ns;
     ls->done = status == 0 ? 1 : 0;
     amcd_out& o = O[blockIdx.x];
@@ -1418,7 +1556,8 @@ static int alloc_active(amcd_batch* b, Inst& d, int acap) {
       dalloc(b, &d.aScan, c) || dalloc(b, &d.gAct, c) || dalloc(b, &d.gScan, c) || dalloc(b, &d.gExt, c) ||
       dalloc(b, &d.gVhat, c) || dalloc(b, &d.gN, c) || dalloc(b, &d.gDead, c) || dalloc(b, &d.gGrow, c) ||
       dalloc(b, &d.cD, c * AMCD_NB) || dalloc(b, &d.cF, c * AMCD_NB) || dalloc(b, &d.cR, c * AMCD_NB) ||
-      dalloc(b, &d.cTgt, c * AMCD_NB) || dalloc(b, &d.cK, c * AMCD_NB))
+      dalloc(b, &d.cTgt, c * AMCD_NB) || dalloc(b, &d.cK, c * AMCD_NB) || dalloc(b, &d.aRN, c * AMCD_NB) ||
+      dalloc(b, &d.gR, c * 9) || dalloc(b, &d.gC, c * 3) || dalloc(b, &d.gV, c) || dalloc(b, &d.gG, c) || dalloc(b, &d.cV, c * AMCD_NB))
     return -1;
   return 0;
 }
@@ -1478,7 +1617,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     if (dalloc(b, &d.vs, Nn) || dalloc(b, &d.gid, Nn) || dalloc(b, &d.ext, Nn) || dalloc(b, &d.cx, Nn) ||
         dalloc(b, &d.cy, Nn) || dalloc(b, &d.cz, Nn) || dalloc(b, &d.T, Nn) || dalloc(b, &d.bj, Nn) ||
         dalloc(b, &d.amap, Nn) || dalloc(b, &d.bcA, (size_t)d.nblkZ) || dalloc(b, &d.bcM, (size_t)d.nblkZ) ||
-        dalloc(b, &d.ls, 1)) {
+        dalloc(b, &d.zliq, (size_t)p[i].nz) || dalloc(b, &d.ls, 1)) {
       snprintf(err, errlen, "%s", g_dev_err.c_str());
       amcd_destroy(b);
       return -1;
@@ -1491,6 +1630,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     hipMemsetAsync(d.cz, 0, Nn * 8, b->stream);
     hipMemsetAsync(d.T, 0, Nn * 8, b->stream);
     hipMemsetAsync(d.bj, 0, Nn, b->stream);
+    hipMemsetAsync(d.zliq, 0xff, (size_t)p[i].nz * 4, b->stream);  // unknown: every plane may hold liquid
     hipMemsetAsync(d.amap, 0xff, Nn * 4, b->stream);
     hipMemsetAsync(d.ls, 0, sizeof(LoopState), b->stream);
     int acap = (int)std::min<long long>(N, std::max<long long>(1 << 16, N / 8));
@@ -1554,6 +1694,7 @@ int amcd_upload(amcd_batch* b, int i, const int* gid, const int* vstate, const d
   if (vstate) {
     for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(vstate[v] & 3);
     HIPT(hipMemcpy(d.vs, tmp.data(), N, hipMemcpyHostToDevice));
+    HIPT(hipMemset(d.zliq, 0xff, (size_t)d.p.nz * 4));
   }
   if (bj) {
     for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(bj[v] ? 1 : 0);
@@ -1700,6 +1841,7 @@ static int launch_update(amcd_batch* b, bool fused_temperature) {
   hipLaunchKernelGGL(k_solid_to_mushy, dim3(b->bstartG[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, n,
                      b->d_steps);
   tmark(b, 2);
+  hipLaunchKernelGGL(k_zliq_clear, dim3(n), dim3(256), 0, b->stream, b->d_inst, b->d_steps);
   hipLaunchKernelGGL(k_active_flags, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
                      b->d_steps);
   hipLaunchKernelGGL(k_scan, dim3(n), dim3(1024), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
@@ -1921,6 +2063,7 @@ int amcd_write_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, co
   if (cnt) HIPT(hipMemcpyAsync(dg, gtmp, sizeof(int) * cnt, hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_write_layer, dim3((d.N + 255) / 256), dim3(256), 0, b->stream, b->d_inst, i, iz1, ilaserLoc,
                      gbase, dg);
+  HIPT(hipMemsetAsync(d.zliq, 0xff, (size_t)d.p.nz * 4, b->stream));
   HIPT(hipStreamSynchronize(b->stream));
   HIPT(hipFree(dg));
   return 0;

@@ -101,6 +101,8 @@ static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<a
     for (int i = 0; i < n; ++i) {
       if (out[i].status < 0 && !st[i].skip) {
         if (out[i].overflow) return fail(AMCAFE_ERR_DEVICE, "active list capacity could not be grown");
+        if (out[i].status == -3)
+          return fail(AMCAFE_ERR_STATE, "the interface has more than 2^19 active cells (capture-loop limit)");
         return fail(AMCAFE_ERR_STATE, "device step failed");
       }
       if (out[i].status != 1) continue;

@@ -14,6 +14,9 @@ Also reported, in the same JSON line:
                 duration (HIP events on the library's stream), against 8 TB/s HBM3E
   cpu_baseline  the oracle (a one-rank CPU port of the reference path) timed on the
                 host for a few steps starting from the GPU's state before the timed region
+  batched       the 64 decks of decks/c2_batched (64^3 .. 256x128x96 voxels, one track,
+                1-2 layers) run to completion as one batch and then one by one: aggregate
+                cell-updates/s of each and the ratio
 """
 import argparse
 import json
@@ -51,6 +54,43 @@ def algorithmic_bytes(stage, nvox, st):
     return None
 
 
+def batched_vs_single(deck_dir):
+    """The batched workload: every deck of deck_dir run to completion in one Sim (one launch per
+    stage for all instances), then each deck alone in its own Sim, one after the other.  Host
+    set-up (deck parse, base plate, first layer) is outside both timed regions."""
+    import glob
+    import amcafe_amd
+    names = sorted(glob.glob(os.path.join(deck_dir, "*.in")))
+    decks = [amcafe_amd.read_deck(n) for n in names]
+    sim = amcafe_amd.Sim(decks)
+    nv = []
+    t0i = []
+    for i in range(len(decks)):
+        inf = sim.info(i)
+        nv.append(inf["nX"][0] * inf["nX"][1] * inf["nX"][2])
+        t0i.append(inf["tInd"])
+    t0 = time.perf_counter()
+    bsteps = sim.run()
+    tb = time.perf_counter() - t0
+    upd = sum(nv[i] * (sim.info(i)["tInd"] - t0i[i]) for i in range(len(decks)))
+    sim.close()
+    ts, sup = 0.0, 0
+    for d in decks:
+        one = amcafe_amd.Sim(d)
+        inf = one.info(0)
+        t0 = time.perf_counter()
+        one.run()
+        ts += time.perf_counter() - t0
+        sup += inf["nX"][0] * inf["nX"][1] * inf["nX"][2] * (one.info(0)["tInd"] - inf["tInd"])
+        one.close()
+    assert sup == upd
+    return {"workload": os.path.basename(deck_dir.rstrip("/")), "instances": len(decks), "voxels": int(sum(nv)),
+            "cell_updates": int(upd), "batched_steps": bsteps,
+            "batched_value": round(upd / tb, 1), "batched_s": round(tb, 3),
+            "sequential_value": round(sup / ts, 1), "sequential_s": round(ts, 3),
+            "speedup": round(ts / tb, 3), "unit": "cell-updates/s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -59,6 +99,7 @@ def main():
     ap.add_argument("--workload", default="c1_raster", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-single comparison")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scale for HBM counters)")
     args = ap.parse_args()
@@ -141,6 +182,10 @@ def main():
                "sample": f"{args.cpu_steps} steps of {args.workload} from the state after {args.warmup} warmup "
                          f"steps, one thread, {tc:.1f} s"}
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_vs_single(os.path.join(ROOT, "decks", "c2_batched"))
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": round(1e3 * wall / steps, 4), "higher_is_better": True,
@@ -151,7 +196,7 @@ def main():
                    "events_per_step": round(tot["n_events"] / steps, 1),
                    "active_cells_per_step": round(tot["n_active"] / steps, 1),
                    "growing_cells_per_step": round(tot["n_growing"] / steps, 1)},
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

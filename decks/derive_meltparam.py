@@ -31,5 +31,32 @@ def dims(P,v,eta=0.35,sig=50e-6,tL=1733.0):
     hw,xw=best
     zs=np.linspace(1e-7,200e-6,201); Tz=np.array([T(xw,0,z,P,v,eta,sig) for z in zs]); d=zs[Tz>=tL].max() if (Tz>=tL).any() else 0
     return front,tail,hw,d
-for name,P,v in [("c0",200,0.8),("c1",250,1.0)]:
-    print(name, ["%.1f um"%(1e6*q) for q in dims(P,v)])
+def _edge(f, a, b, tol=1e-7):
+    """Bisection for the boundary between f(a) True and f(b) False."""
+    while abs(b - a) > tol:
+        m = .5 * (a + b)
+        if f(m):
+            a = m
+        else:
+            b = m
+    return a
+def dims_bisect(P,v,eta=0.35,sig=50e-6,tL=1733.0):
+    """Same extents as dims() by bisection on the isosurface (a few hundred quadratures)."""
+    z0=1e-9
+    hot=lambda x,y,z: T(x,y,z,P,v,eta,sig)>=tL
+    xs=np.linspace(-800e-6,200e-6,251)
+    h=[x for x in xs if hot(x,0,z0)]
+    if not h: return None
+    front=_edge(lambda x: hot(x,0,z0), max(h), max(h)+4e-6)
+    tail=-_edge(lambda x: hot(x,0,z0), min(h), min(h)-4e-6)
+    best=(0,0)
+    for x in np.linspace(-tail,front,40):
+        if hot(x,0,z0):
+            w=_edge(lambda y: hot(x,y,z0), 0.0, 400e-6)
+            if w>best[0]: best=(w,x)
+    hw,xw=best
+    d=_edge(lambda z: hot(xw,0,z), 1e-7, 400e-6)
+    return front,tail,hw,d
+if __name__ == "__main__":
+    for name,P,v in [("c0",200,0.8),("c1",250,1.0)]:
+        print(name, ["%.1f um"%(1e6*q) for q in dims(P,v)])

@@ -442,10 +442,12 @@ struct Sim {
     gID.assign(N, 0); vState.assign(N, 0); Build.assign(N, 0);
     extents.assign(N, 0.0); centroidOct.assign(3 * (size_t)N, 0.0);
     genlayer.seed(seed1);
-    if (bare) return 0;  // state, grains and laser are set by the caller
-    initialize_voxels();
     // driver (S/main1.C:59-61)
     nlayerTot = int(ceil((double)(g.nX[2] - Nzh) / (double)g.nZlayer));
+    // bare: state, grains and laser are set by the caller (the powder-layer engine stays
+    // at its seed, so a bare instance is exact only until its next AddLayer1)
+    if (bare) return 0;
+    initialize_voxels();
     add_layer1();
     g.update_laser();
     return 0;
@@ -1046,6 +1048,20 @@ void orc_stats(void* h, long* o, double* tinc) {
   o[0] = s->st.Na; o[1] = s->st.n_mushy0; o[2] = s->st.iters; o[3] = s->st.captures;
   o[4] = s->st.nucleations; o[5] = s->st.draws;
   if (tinc) *tinc = s->st.tinc;
+}
+// laser inputs of the next temperature evaluation, as TempField.C:146-171 forms them:
+// x, y, z (grid frame), dirp, gth, ilaserLoc
+void orc_laser_inputs(void* h, double* o) {
+  orc::Sim* s = (orc::Sim*)h;
+  const orc::Grid& g = s->g;
+  const std::vector<int>& isv = s->ispvec;
+  o[0] = g.lcoor[2 * isv[g.isp]] - g.offset[0];
+  o[1] = g.lcoor[2 * isv[g.isp] + 1] - g.offset[1];
+  o[2] = g.ilaserLoc * g.dX[2] - g.offset[2];
+  if (fmod(g.isp, g.Nsd) == 0) o[3] = (g.lcoor[2 * isv[g.isp + 1]] - g.lcoor[2 * isv[g.isp]]);
+  else o[3] = (g.lcoor[2 * isv[g.isp]] - g.lcoor[2 * isv[g.isp - 1]]);
+  o[4] = g.gth;
+  o[5] = g.ilaserLoc;
 }
 // event log of the last UpdateVoxels (grid indices); returns the number of events
 long orc_events(void* h, int* vox, int* src, double* dt, long cap) {

@@ -35,6 +35,7 @@ def _load(path):
     L.orc_stats.argtypes = [_P, _P, _P]
     L.orc_events.argtypes = [_P, _P, _P, _P, ctypes.c_long]
     L.orc_events.restype = ctypes.c_long
+    L.orc_laser_inputs.argtypes = [_P, _P]
     L.orc_generate_samples.argtypes = [_I, ctypes.c_uint, _P]
     L.orc_uniform_stream.argtypes = [ctypes.c_ulong, _I, _P]
     L.orc_rot_mat.argtypes = [ctypes.c_double, _P, _P]
@@ -138,6 +139,12 @@ class Oracle:
         vox, src, dt = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
         self.L.orc_events(self.h, vox.ctypes.data, src.ctypes.data, dt.ctypes.data, n)
         return vox, src, dt
+
+    def laser_inputs(self):
+        """(x, y, z, dirp, gth, ilaserLoc) of the next temperature evaluation."""
+        o = np.zeros(6)
+        self.L.orc_laser_inputs(self.h, o.ctypes.data)
+        return o[0], o[1], o[2], o[3], o[4], int(o[5])
 
     def done(self):
         return self.L.orc_done(self.h)

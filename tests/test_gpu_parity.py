@@ -80,3 +80,72 @@ def test_c0_single_track_full_run():
     """Workload c0 (128x64x32, one track, one layer): every step bit-identical."""
     n = _lockstep("c0_single_track.in", 10**6)
     assert n >= 128
+
+
+def _deck_values(text):
+    kv = {}
+    for line in text.split("\n"):
+        p = line.split("#")[0].split()
+        if p:
+            kv[p[0].lower()] = p[1:]
+    return kv
+
+
+def test_seam_level_api_per_step():
+    """The seam entry points (amcafe_domain_*: upload, AnalyticTempCurr, UpdateVoxels,
+    download, grains) against the oracle, one step at a time from the oracle's state,
+    over the whole small deck including its layer changes."""
+    deck = read("t_small.in")
+    kv = _deck_values(deck)
+    nX = [int(v) for v in kv["nx"]]
+    tL = float(kv["tl"][0])
+    dom = amcafe_amd.Domain(nX, float(kv["dx"][0]), tL, float(kv["ts"][0]), 300.0,
+                            [float(v) for v in kv["meltparam"]], float(kv["bmv"][0]), float(kv["bmdelt"][0]),
+                            float(kv["r"][0]), float(kv["dendritegrowth2"][0]), float(kv["dendritegrowth2"][1]))
+    o = Oracle(deck, crmath=True)
+    events = 0
+    for step in range(400):
+        f = o.fields()
+        s = o.scalars()
+        dom.upload(gID=f["gID"], vState=f["vState"], extents=f["extents"], centroidOct=f["centroidOct"],
+                   BuildjID=f["BuildjID"])
+        dom.set_grains(o.grains())
+        x, y, z, dirp, gth, il = o.laser_inputs()
+        any_melt = dom.temperature(x, y, z, dirp, gth, il)
+        o.temperature()
+        st = dom.update_voxels(int(s["tInd"]), il)
+        o.update_voxels()
+        g, of = dom.download(), o.fields()
+        assert any_melt == int(np.any(of["TempCurr"] >= tL)), f"step {step}: end-of-scan flag"
+        for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
+            assert _same(g[k], of[k]), f"seam step {step}: " + _diff(k, g[k], of[k])
+        assert np.array_equal(dom.grains(), o.grains()), f"seam step {step}: grain table"
+        so = o.stats()
+        assert st["n_events"] == so["n_events"] and st["n_nucleations"] == so["n_nucleations"]
+        events += st["n_events"]
+        if o.finish_step():
+            break
+    assert events > 100
+
+
+def test_c1_full_size_lockstep_from_mid_run_state():
+    """Workload c1 at full size (512x512x128): the GPU's state after 150 steps, taken
+    over by the correctly rounded oracle; the next steps must agree field for field."""
+    deck = read("c1_raster.in")
+    sim = amcafe_amd.Sim(deck)
+    for _ in range(150):
+        sim.step()
+    f = sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID"))
+    o = Oracle(deck, crmath=True, bare=True)
+    o.load_state(f, sim.grains(0), sim.info(0))
+    for n in range(2):
+        sim.step()
+        o.temperature()
+        o.update_voxels()
+        o.finish_step()
+        g, of = sim.fields(0), o.fields()
+        for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
+            assert _same(g[k], of[k]), f"c1 step {151 + n}: " + _diff(k, g[k], of[k])
+        s, so = sim.stats(0), o.stats()
+        assert s["n_events"] == so["n_events"] > 100
+    assert np.array_equal(sim.grains(0), o.grains())

@@ -273,3 +273,27 @@ def test_glibc_and_correctly_rounded_runs_agree_on_grain_fields():
         if da or db:
             assert da and db
             break
+
+
+def test_bare_oracle_takes_over_a_running_state():
+    """The hand-over the full-size GPU test and the CPU baseline use: a bare oracle loaded
+    with another run's fields, grain table and laser scalars continues it bit for bit
+    (it must not run the end-of-build CleanLayer of S/main1.C:94 before the last layer)."""
+    deck = read("c0_single_track.in")
+    a = Oracle(deck, crmath=True)
+    for _ in range(60):
+        a.step()
+    sc = a.scalars()
+    info = {k: sc[k] for k in ("time", "gth")}
+    info.update({k: int(sc[k]) for k in ("isp", "ilaserLoc", "tInd", "indlayer", "inewscanflg", "inewlayerflg")})
+    b = Oracle(deck, crmath=True, bare=True)
+    b.load_state(a.fields(), a.grains(), info)
+    for _ in range(2):
+        a.step()
+        b.temperature()
+        b.update_voxels()
+        b.finish_step()
+        fa, fb = a.fields(), b.fields()
+        for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
+            assert np.array_equal(fa[k], fb[k], equal_nan=fa[k].dtype.kind == "f"), k
+        assert np.array_equal(a.grains(), b.grains())
