@@ -54,6 +54,8 @@ SIGNATURES = {
     "amcafe_domain_set_grains": (_I, [_P, _P, _I]),
     "amcafe_domain_get_grains": (_I, [_P, _P, ctypes.POINTER(_I)]),
     "amcafe_analytic_temp_curr": (_I, [_P, ctypes.POINTER(Laser), ctypes.POINTER(_I)]),
+    "amcafe_easm_temp_curr": (_I, [_P, ctypes.POINTER(Laser), _D, _D, ctypes.POINTER(_I),
+                                   ctypes.POINTER(ctypes.c_longlong)]),
     "amcafe_update_voxels": (_I, [_P, _I, _I, ctypes.POINTER(Stats)]),
     "amcafe_sim_create": (_I, [ctypes.POINTER(_C), _I, ctypes.POINTER(_P)]),
     "amcafe_sim_destroy": (None, [_P]),
@@ -63,6 +65,9 @@ SIGNATURES = {
     "amcafe_sim_get_fields": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "amcafe_sim_get_grains": (_I, [_P, _I, _P, ctypes.POINTER(_I)]),
     "amcafe_sim_write_hdf5": (_I, [_P, _I, _C]),
+    "amcafe_sim_set_output": (_I, [_P, _I, _C]),
+    "amcafe_write_hdf5_fields": (_I, [_C, _P, _P, _P, _P, _P, _P, _P, _I]),
+    "amcafe_ipf_colours": (_I, [_P, _I, _P]),
     "amcafe_sim_set_timing": (_I, [_P, _I]),
     "amcafe_sim_timings": (_I, [_P, _P, _I]),
 }

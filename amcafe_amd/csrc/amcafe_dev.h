@@ -35,6 +35,7 @@ typedef struct {
   double vhat_tS, vhat_T0;  // Avel*pow(tL-T,nvel) at the two sub-liquidus analytic temperatures
   double Avel, nvel;
   unsigned int seed1;  // VoxelsCA::seed1
+  double bmV;          // beam speed (EASM source path)
 } amcd_params;
 
 // per-step inputs (host computes them from the Grid / TempField state)
@@ -94,8 +95,19 @@ int amcd_resume(amcd_batch* b, const amcd_step* steps, amcd_out* outs);
 // per-layer support (S/VoxelsCA.C:800-839, 887-982)
 int amcd_grain_presence(amcd_batch* b, int i, int ngrain, int* present);
 int amcd_remap_grains(amcd_batch* b, int i, const int* remap, int ngrain);
-int amcd_write_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, const int* gtmp);
+// base plate (S/BasePlate.C:41-97 + S/VoxelsCA.C:125-161): Voronoi of the nzh bottom planes
+// over ng sites (xyz interleaved), gID = site + 1, vState 3 (2 in the top plate plane),
+// centroids at voxel centres; the rest of the domain is left as created (empty)
+int amcd_init_plate(amcd_batch* b, int i, int nzh, const double* xs, int ng);
+// AddLayer1's Voronoi, renumbering and voxel writes (S/VoxelsCA.C:920-981): the layer
+// iz1..ilaserLoc-1 takes grains gbase + 1..; *n_used = sites that label a voxel
+int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, const double* xs, int ng, int* n_used);
 
+// EASM temperature (S/TempField.C:79-137) instead of the analytic one: beam power Q,
+// layer thickness; outs[i].any_melt = 1 when some voxel is below tL; *n_evals (may be
+// NULL) = source-integral evaluations summed over the batch
+int amcd_easm(amcd_batch* b, const amcd_step* steps, double Q, double layerT, amcd_out* outs,
+              unsigned long long* n_evals);
 // temperature + UpdateVoxels in one pass (SetLiquid3 fused into the temperature kernel)
 int amcd_step_fused(amcd_batch* b, const amcd_step* steps, amcd_out* outs);
 // grain-table capacity (grains + nucleation pool) for the coming step

@@ -31,6 +31,7 @@
 
 #include "amcafe_dev.h"
 #include "ddmath.h"
+#include "gl25.h"
 
 // Phase timers of the capture loop (core-clock cycles summed over events), compiled in
 // only with -DAMCD_PHASE_PROFILE; read with amcd_phase_profile().
@@ -97,6 +98,8 @@ struct Inst {
   double *gExt, *gVhat;
   double *gR, *gC;  // per slot: the cell's grain rotation (9) and centroid (3)
   int *gV, *gG;     // per slot: the cell's voxel and grain
+  float2* gLH;      // per slot: capture-clock bounds (lo, hi) of the memory-resident tier
+  int2* gHd;        // per slot: list head (entry, target) of the memory-resident tier
   uint8_t *gN, *gDead, *gGrow;
   double *cD, *cF, *cR;
   int *cTgt, *cV;   // per entry: target (active index) and its voxel
@@ -216,6 +219,117 @@ __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, 
     d.vs[v] = s | (hot ? 0x80 : 0);
   }
   if (__any(hot) && (threadIdx.x & 63) == 0) atomicOr(&O[ii].any_melt, 1);
+}
+
+// -------------------------------------------------------------------------------------
+// TempField::EASM (S/TempField.C:79-137): the temperature of a Gaussian moving source with
+// 316L properties, Utilities::EASM_Temp_LG (S/Utilities.C:128-140).  Every voxel is an
+// independent FP64 fixed-point problem (FP64-VALU bound).
+__constant__ double kGLx[AMC_GL25_N] = AMC_GL25_X;
+__constant__ double kGLw[AMC_GL25_N] = AMC_GL25_W;
+
+// Utilities::Materials_316L (S/Utilities.C:60-96): cP, k, rho, alpha at temperature T
+// (pow(10, -n) written as the literal it folds to)
+__device__ static inline void mat316(double T, double& cP, double& k, double& rho, double& alpha) {
+  if (T < 1660)
+    rho = 8084.2 - 0.42086 * T - 3.8942 * 1e-5 * T * T;
+  else if (T >= 1660 && T < 3090)
+    rho = 7432.7 + 3.9338 * 1e-2 * T - 1.8007 * 1e-4 * T * T;
+  else
+    rho = 6124.72;
+  cP = T < 1660 ? 458.984 + 0.1328 * T : 769.855;
+  if (T < 1660)
+    k = 9.248 + 1.571 * 1e-2 * T;
+  else if (T >= 1660 && T < 3090)
+    k = 12.41 + 3.279 * 1e-3 * T;
+  else
+    k = 21.6465;
+  alpha = k / (rho * cP);
+}
+
+// Utilities::Integral (S/Utilities.C:105-124): 300 + the 25-point Gauss-Legendre sum over
+// s in [0, t] of Temp_Fun_G (S/Utilities.C:228-249), t = 8e-4 s, sigma 50 um, eta 0.56,
+// properties at the iterate T.  Nodes x = t/2 + (t/2) xi, weights (t/2) w, summed in order.
+__device__ static inline double easm_integral(const double L[3], double T, const double R[3], double Q, double bmV,
+                                              int* evals) {
+  const double t = 80.0e-5, sigma = 50.0e-6, eta = 0.56;
+  double cP, k, rho, alpha;
+  mat316(T, cP, k, rho, alpha);
+  const double C = eta * Q / (M_PI * rho * cP * sqrt(4 * M_PI * alpha));
+  const double x0 = L[0] - bmV * t;
+  const double slp = (t - 0.0) / 2.0, shft = (0.0 + t) / 2.0;
+  double sum = 0.0;
+  for (int i = 0; i < AMC_GL25_N; ++i) {
+    const double x = shft + slp * kGLx[i];
+    const double w = kGLw[i] * slp;
+    const double Lx = x0 + bmV * x;
+    const double dx = R[0] - Lx, dy = R[1] - L[1], dz = fabs(R[2] - L[2]);
+    const double f = C * pow(t - x, -0.5) / (2 * alpha * (t - x) + sigma * sigma) *
+                     exp(-1 * (dx * dx + dy * dy) / (4 * alpha * (t - x) + 2 * (sigma * sigma)) -
+                         dz * dz / (4 * alpha * (t - x)));
+    sum += w * f;
+  }
+  ++*evals;
+  return 300 + sum;
+}
+
+// Utilities::EASM_Temp_LG (S/Utilities.C:128-140): damped Picard iteration from T
+__device__ static inline double easm_temp(const double L[3], const double R[3], double T, double Q, double bmV,
+                                          int* evals) {
+  double Told = T;
+  int n = 0;
+  const int nlim = 100;
+  const double tol = 10;
+  double Tnew = easm_integral(L, T, R, Q, bmV, evals);
+  while (fabs(Told - Tnew) > tol && n < nlim) {
+    Told = Tnew;
+    const double T_temp = easm_integral(L, Told, R, Q, bmV, evals);
+    Tnew = 0.5 * (Told + T_temp);
+    n++;
+  }
+  return Tnew;
+}
+
+// TempField::EASM's voxel loop (S/TempField.C:90-115): 0 K above the laser plane, 0.99 T0
+// ahead of the laser in the top layer, EASM_Temp_LG from 0 K elsewhere; BuildjID as the
+// analytic model sets it; the "T >= tL" flag for SetLiquid3; O.any_melt collects whether
+// some voxel stayed below tL (the all_of test of S/TempField.C:121-124 is its negation).
+__global__ __launch_bounds__(GB) void k_easm(Inst* I, const int* bstart, int ninst, const amcd_step* S, amcd_out* O,
+                                             double Q, double layerT, unsigned long long* n_evals) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  const amcd_step& st = S[ii];
+  if (st.skip) return;
+  const int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
+  bool cold = false;
+  int evals = 0;
+  if (v < d.N) {
+    int i, j, k;
+    decode(d, v, i, j, k);
+    const amcd_params& p = d.p;
+    const double x0 = ((double)i + .5) * p.dx;
+    const double y0 = ((double)j + .5) * p.dy;
+    const double zp = ((double)k + .5) * p.dz;
+    double T = 0.0;
+    if (!(zp > st.lz)) {
+      const double xp = st.cg * (x0 - p.hx) + st.sg * (y0 - p.hy) + p.hx;
+      const double yp = (-st.sg) * (x0 - p.hx) + st.cg * (y0 - p.hy) + p.hy;
+      const double R[3] = {xp, yp, zp}, L[3] = {st.lx, st.ly, st.lz};
+      if ((x0 > st.lx) && (zp > (st.lz - layerT)))
+        T = .99 * p.T0;
+      else
+        T = easm_temp(L, R, 0.0, Q, p.bmV, &evals);
+      if (T > p.tL) d.bj[v] = 1;
+      else if (T < p.T0) d.bj[v] = 0;
+    }
+    d.T[v] = T;
+    const bool hot = T >= p.tL;
+    cold = !hot;
+    d.vs[v] = (d.vs[v] & 3) | (hot ? 0x80 : 0);
+  }
+  if (__any(cold) && (threadIdx.x & 63) == 0) atomicOr(&O[ii].any_melt, 1);
+  for (int m = 32; m >= 1; m >>= 1) evals += __shfl_xor(evals, m);
+  if ((threadIdx.x & 63) == 0 && evals) atomicAdd(n_evals, (unsigned long long)evals);
 }
 
 // SetLiquid3 (S/VoxelsCA.C:1098-1116) when the temperature kernel ran without it.
@@ -910,23 +1024,34 @@ struct Decision {
 template <int NW>
 __device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX, int nd) {
   Best e[NW];
+  int wi[NW];
 #pragma unroll
-  for (int q = 0; q < NW; ++q) e[q] = sh.sb[par][q];
+  for (int q = 0; q < NW; ++q) {
+    e[q] = sh.sb[par][q];
+    wi[q] = q;
+  }
   const double U = sh.ubuf[(nd >> 6) & 1][nd & 63];
   const int susp = sh.susp[par];
-  Best b = e[0];
-  int wq = 0;
+  // pairwise tree, the left entry kept unless the right one is strictly better: the
+  // first best entry in wave order, as a sequential scan finds it (the order is a
+  // strict weak order: times and ruqs are never NaN here)
 #pragma unroll
-  for (int q = 1; q < NW; ++q) {
-    const bool bt = better(e[q], b);
-    b.t = bt ? e[q].t : b.t;
-    b.r = bt ? e[q].r : b.r;
-    b.key = bt ? e[q].key : b.key;
-    b.cand = bt ? e[q].cand : b.cand;
-    b.act = bt ? e[q].act : b.act;
-    b.tgt = bt ? e[q].tgt : b.tgt;
-    wq = bt ? q : wq;
+  for (int h = 1; h < NW; h <<= 1) {
+#pragma unroll
+    for (int q = 0; q + h < NW; q += 2 * h) {
+      const Best& o = e[q + h];
+      const bool bt = better(o, e[q]);
+      e[q].t = bt ? o.t : e[q].t;
+      e[q].r = bt ? o.r : e[q].r;
+      e[q].key = bt ? o.key : e[q].key;
+      e[q].cand = bt ? o.cand : e[q].cand;
+      e[q].act = bt ? o.act : e[q].act;
+      e[q].tgt = bt ? o.tgt : e[q].tgt;
+      wi[q] = bt ? wi[q + h] : wi[q];
+    }
   }
+  const Best b = e[0];
+  const int wq = wi[0];
   Decision x;
   x.stop = 0;
   x.nuc = 0;
@@ -995,6 +1120,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   x.i1 = sh.r_i1;
   x.dt = 0.0;
   bool apply_first = go == 2;
+  PHASE_INIT(ot);
   while (go > 0) {
     // The slot numbers are re-derived from an opaque copy of the thread index in every
     // iteration: otherwise the compiler keeps per-cell addresses live across the loop,
@@ -1059,11 +1185,14 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       }
       b = wave_best(b);
       if (lane == 0) sh.sb[par][w] = b;
+      PHASE(ot, 7, tid == 64);
       __syncthreads();
+      PHASE(ot, 8, tid == 64);
       // ---- DECIDE
       x = decide<NW>(sh, par, T, p.DelT, p.rX, nd);
       if (x.stop) break;
       ++nd;
+      PHASE(ot, 5, tid == 64);
     }
     apply_first = false;
     // ---- APPLY
@@ -1123,6 +1252,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
     }
     skip = i1;
     par ^= 1;
+    PHASE(ot, 6, tid == 64);
   }
   // half-diagonals back to the growing list (the cell still in the mailbox is current there)
   if (loaded) {
@@ -1136,10 +1266,159 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   }
 }
 
+// The owner protocol of owner_loop for growing lists larger than the register tiers hold:
+// each cell's state (E in gExt, V in gVhat, bounds in gLH, head in gHd) stays in global
+// memory and every event streams the lane's cells (slots me, me + NO, ...).  Same
+// decisions, same arithmetic, same order of additions as the register tiers.
+template <int NT>
+__device__ static void owner_loop_mem(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
+                                      long long& visits, long long& evals) {
+  constexpr int NO = NT - 64;
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int me = tid - 64;
+  const float fInf = __builtin_inff();
+  const amcd_params& p = d.p;
+  int ns = go != -2 ? sh.nslots : 0;  // slots this lane knows of (new ones arrive by mailbox)
+  for (int s = me; s < ns; s += NO) {
+    const SlotHead h = slot_head(d, dead, s, 0, -1);
+    float lo = fInf, hi = fInf;
+    if (h.tgt1 >= 0) tau_bounds(h.D1, d.gExt[s], d.gVhat[s], T, lo, hi);
+    d.gHd[s] = make_int2(h.c1, h.tgt1);
+    d.gLH[s] = make_float2(lo, hi);
+  }
+  int skip = -1;
+  int par = 0;
+  int nd = 0;
+  int nev = 0;
+  Decision x;
+  x.stop = 0;
+  x.nuc = 1;
+  x.i1 = sh.r_i1;
+  x.dt = 0.0;
+  bool apply_first = go == 2;
+  while (go > 0) {
+    if (!apply_first) {
+      // ---- EVAL: the wave's smallest upper bound, then the cells whose lower bound reaches it
+      float u = fInf, lmin = fInf;
+      for (int s = me; s < ns; s += NO) {
+        const float2 lh = d.gLH[s];
+        u = fminf(u, lh.y);
+        lmin = fminf(lmin, lh.x);
+      }
+      for (int m = 32; m >= 1; m >>= 1) u = fminf(u, __shfl_xor(u, m));
+      Best b = best_none();
+      if (lmin <= u) {
+        for (int s = me; s < ns; s += NO) {
+          const float lo = d.gLH[s].x;
+          if (!(lo != fInf && lo <= u)) continue;
+          const int2 hd = d.gHd[s];
+          const int c = hd.x;
+          const int pos = c - s * AMCD_NB;
+          const double e = d.gExt[s], v = d.gVhat[s];
+          const double D1 = d.cD[c];
+          const double Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
+          const double rr = d.cR[c];
+          const int kb = d.gScan[s] * 32;
+          const int kk = d.cK[c];
+          const int a = d.gAct[s];
+          visits++;
+          evals++;
+          const double t1 = (D1 - e) / v;
+          if (t1 <= 1e6) {
+            Best y;
+            y.t = t1;
+            y.r = rr;
+            y.key = kb + kk;
+            y.cand = c;
+            y.act = a;
+            y.tgt = hd.y;
+            if ((Dn - e) / v == t1) {
+              const int n = d.gN[s];
+              for (int q = pos + 1; q < n; ++q) {
+                const int tq = d.cTgt[c - pos + q];
+                if (!live_target(dead, tq, skip)) continue;
+                evals++;
+                const double tt = (d.cD[c - pos + q] - e) / v;
+                if (!(tt == t1)) break;
+                Best z;
+                z.t = tt;
+                z.r = d.cR[c - pos + q];
+                z.key = kb + d.cK[c - pos + q];
+                z.cand = c - pos + q;
+                z.act = a;
+                z.tgt = tq;
+                if (better(z, y)) y = z;
+              }
+            }
+            if (better(y, b)) b = y;
+          }
+        }
+      }
+      b = wave_best(b);
+      if (lane == 0) sh.sb[par][w] = b;
+      __syncthreads();
+      x = decide<NW>(sh, par, T, p.DelT, p.rX, nd);
+      if (x.stop) break;
+      ++nd;
+    }
+    apply_first = false;
+    // ---- APPLY
+    const double Dt = x.dt;
+    const int i1 = x.i1;
+    // the cell created by the previous event joins its owner's slots ungrown; the pass
+    // below grows it like every other cell and computes its bounds
+    const NewCell& nc = sh.mb[par];
+    if (nc.valid && nc.slot % NO == me) {
+      d.gExt[nc.slot] = nc.E;
+      d.gVhat[nc.slot] = nc.V;
+      d.gHd[nc.slot] = make_int2(nc.c1, nc.tgt1);
+      d.gLH[nc.slot] = make_float2(-fInf, fInf);
+    }
+    if (nc.valid) ns = max(ns, nc.slot + 1);
+    T = T + Dt;
+    ++nev;
+    const bool all = nev >= (1 << 20);
+    if (all) nev = 0;
+    for (int s = me; s < ns; s += NO) {
+      int2 hd = d.gHd[s];
+      if (hd.y < 0) continue;
+      float2 lh = d.gLH[s];
+      double e = d.gExt[s];
+      const double v = d.gVhat[s];
+      if (!x.nuc) {  // growth (S/VoxelsCA.C:511-516)
+        const double g = e + v * Dt;
+        if (g < 0.0) {
+          e = 0.0;
+          lh.x = -fInf;
+        } else {
+          e = g;
+        }
+        d.gExt[s] = e;
+      }
+      if (hd.y == i1) {
+        const SlotHead h = slot_head(d, dead, s, hd.x - s * AMCD_NB + 1, i1);
+        hd = make_int2(h.c1, h.tgt1);
+        lh = make_float2(fInf, fInf);
+        if (h.tgt1 >= 0) tau_bounds(h.D1, e, v, T, lh.x, lh.y);
+        d.gHd[s] = hd;
+        d.gLH[s] = lh;
+      } else if (lh.x == -fInf || all) {
+        tau_bounds(d.cD[hd.x], e, v, T, lh.x, lh.y);
+        d.gLH[s] = lh;
+      }
+    }
+    skip = i1;
+    par ^= 1;
+  }
+}
+
 template <int NT, int K>
 __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, amcd_out* O) {
   constexpr int NW = NT / 64;
-  constexpr int CAP = (NT - 64) * K;
+  // slots this tier holds: K per owner thread in registers, or (K = 0) every slot of the
+  // domain's arrays in memory
+  const int CAP = K > 0 ? (NT - 64) * K : I[blockIdx.x].acap;
   __shared__ EngineShared sh;
   __shared__ int s_go;
   __shared__ unsigned s_dead[AMCD_DEAD_BITS / 32];
@@ -1183,7 +1462,10 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const int go = s_go;
   long long visits = 0, evals = 0;
   if (w > 0) {
-    owner_loop<NT, K>(d, sh, s_dead, go, ls->tinc, visits, evals);
+    if constexpr (K > 0)
+      owner_loop<NT, K>(d, sh, s_dead, go, ls->tinc, visits, evals);
+    else
+      owner_loop_mem<NT>(d, sh, s_dead, go, ls->tinc, visits, evals);
   } else {
     // engine state, identical in every lane of wave 0
     double tinc = ls->tinc;
@@ -1485,9 +1767,109 @@ __global__ void k_remap(int* gid, int N, const int* remap, int ngrain) {
   int g = gid[v];
   if (g > 0 && g <= ngrain) gid[v] = remap[g - 1];
 }
-// AddLayer1's voxel writes (S/VoxelsCA.C:952-981): the new layer takes the host's
-// Voronoi labels, everything at or above the laser plane is emptied.
-__global__ void k_write_layer(Inst* I, int ii, int iz1, int ilaserLoc, int gbase, const int* gtmp) {
+// Brute-force Voronoi labelling of a z-slab: each voxel takes the first site (in draw
+// order) at the smallest FP64 squared distance, ((ax*ax + ay*ay) + az*az) with no
+// contraction, the order the reference evaluates it in.
+//   plate (S/BasePlate.C:74-78, 104-117): voxel centres ((i + .5) dx, ...), planes
+//     0..nz-1; the first site is taken unconditionally (start at +inf, strict <).
+//   powder layer (S/VoxelsCA.C:920-933): voxel corners (i dx, j dy, (z0 + k) dz), planes
+//     z0..z0+nz-1 (planes above the domain included, as the reference labels them);
+//     best distance starts at 1e6, strict <.
+// Sites are staged through LDS in chunks; every lane of a wave reads the same site
+// (an LDS broadcast).  Plate mode writes the base-plate fields of S/VoxelsCA.C:125-161
+// directly; layer mode writes the label and marks the site used.
+#define VOR_CHUNK 512
+template <bool LAYER>
+__global__ __launch_bounds__(256) void k_voronoi(Inst* I, int ii, int z0, int nz, const double* __restrict__ xs,
+                                                 int ng, int* __restrict__ label, int* __restrict__ used) {
+  __shared__ double sx[VOR_CHUNK], sy[VOR_CHUNK], sz[VOR_CHUNK];
+  const Inst& d = I[ii];
+  const long long n = (long long)d.nxy * nz;
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i = 0, j = 0, k = 0;
+  if (v < n) {
+    k = (int)(v / d.nxy);
+    const int r = (int)(v - (long long)k * d.nxy);
+    j = r / d.p.nx;
+    i = r - j * d.p.nx;
+  }
+  double x, y, z;
+  if (LAYER) {
+    x = i * d.p.dx;
+    y = j * d.p.dy;
+    z = (z0 + k) * d.p.dz;
+  } else {
+    x = ((double)i + .5) * d.p.dx;
+    y = ((double)j + .5) * d.p.dy;
+    z = ((double)(z0 + k) + .5) * d.p.dz;
+  }
+  double bd = LAYER ? 1e6 : __builtin_huge_val();
+  int best = 0;
+  for (int c0 = 0; c0 < ng; c0 += VOR_CHUNK) {
+    const int m = min(VOR_CHUNK, ng - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < m; q += blockDim.x) {
+      sx[q] = xs[3 * (c0 + q)];
+      sy[q] = xs[3 * (c0 + q) + 1];
+      sz[q] = xs[3 * (c0 + q) + 2];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int q = 0; q < m; ++q) {
+      const double ax = x - sx[q], ay = y - sy[q], az = z - sz[q];
+      const double dd = ax * ax + ay * ay + az * az;
+      const bool lt = dd < bd;
+      bd = lt ? dd : bd;
+      best = lt ? c0 + q : best;
+    }
+  }
+  if (v >= n) return;
+  if (LAYER) {
+    label[v] = best;
+    used[best] = 1;
+  } else {
+    d.gid[v] = best + 1;
+    d.vs[v] = (v < (long long)d.nxy * (nz - 1)) ? 3 : 2;
+    d.cx[v] = x;
+    d.cy[v] = y;
+    d.cz[v] = z;
+  }
+}
+
+// AddLayer1's continuous renumbering (S/VoxelsCA.C:935-947): rank[s] = number of used
+// sites before s; the total goes to *n_used.  One workgroup.
+__global__ __launch_bounds__(1024) void k_site_rank(const int* __restrict__ used, int ng, int* __restrict__ rank,
+                                                    int* __restrict__ n_used) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int c0 = 0; c0 < ng; c0 += 1024) {
+    const int s = c0 + threadIdx.x;
+    const int u = (s < ng && used[s]) ? 1 : 0;
+    int inc = u;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(inc, o);
+      inc += lane >= o ? t : 0;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int before = carry;
+    for (int q = 0; q < w; ++q) before += wsum[q];
+    if (s < ng) rank[s] = before + inc - u;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = before + inc;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *n_used = carry;
+}
+
+// AddLayer1's voxel writes (S/VoxelsCA.C:952-981): the new layer takes the renumbered
+// Voronoi labels as solid grains gbase + 1.., everything at or above the laser plane is
+// emptied.
+__global__ void k_write_layer(Inst* I, int ii, int iz1, int ilaserLoc, int gbase, const int* __restrict__ label,
+                              const int* __restrict__ rank) {
   const Inst& d = I[ii];
   size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= (size_t)d.N) return;
@@ -1495,7 +1877,7 @@ __global__ void k_write_layer(Inst* I, int ii, int iz1, int ilaserLoc, int gbase
   decode(d, (int)v, i, j, k);
   if (k >= iz1 && k < ilaserLoc) {
     int jind = d.nxy * (k - iz1) + d.p.nx * j + i;
-    d.gid[v] = gbase + gtmp[jind];
+    d.gid[v] = gbase + rank[label[jind]] + 1;
     d.vs[v] = 3;
     d.cx[v] = ((double)i + .5) * d.p.dx;
     d.cy[v] = ((double)j + .5) * d.p.dy;
@@ -1557,7 +1939,8 @@ static int alloc_active(amcd_batch* b, Inst& d, int acap) {
       dalloc(b, &d.gVhat, c) || dalloc(b, &d.gN, c) || dalloc(b, &d.gDead, c) || dalloc(b, &d.gGrow, c) ||
       dalloc(b, &d.cD, c * AMCD_NB) || dalloc(b, &d.cF, c * AMCD_NB) || dalloc(b, &d.cR, c * AMCD_NB) ||
       dalloc(b, &d.cTgt, c * AMCD_NB) || dalloc(b, &d.cK, c * AMCD_NB) || dalloc(b, &d.aRN, c * AMCD_NB) ||
-      dalloc(b, &d.gR, c * 9) || dalloc(b, &d.gC, c * 3) || dalloc(b, &d.gV, c) || dalloc(b, &d.gG, c) || dalloc(b, &d.cV, c * AMCD_NB))
+      dalloc(b, &d.gR, c * 9) || dalloc(b, &d.gC, c * 3) || dalloc(b, &d.gV, c) || dalloc(b, &d.gG, c) ||
+      dalloc(b, &d.cV, c * AMCD_NB) || dalloc(b, &d.gLH, c) || dalloc(b, &d.gHd, c))
     return -1;
   return 0;
 }
@@ -1819,6 +2202,32 @@ int amcd_temperature(amcd_batch* b, const amcd_step* steps) {
   return run_temperature(b, 0);
 }
 
+static int read_outs(amcd_batch* b, amcd_out* outs);
+int amcd_easm(amcd_batch* b, const amcd_step* steps, double Q, double layerT, amcd_out* outs,
+              unsigned long long* n_evals) {
+  if (put_steps(b, steps)) return -1;
+  HIPT(hipMemsetAsync(b->d_outs, 0, sizeof(amcd_out) * b->n, b->stream));
+  unsigned long long* dn = nullptr;
+  HIPT(hipMalloc(&dn, sizeof(unsigned long long)));
+  HIPT(hipMemsetAsync(dn, 0, sizeof(unsigned long long), b->stream));
+  tmark(b, 0);
+  hipLaunchKernelGGL(k_easm, dim3(b->bstartG[b->n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, b->n,
+                     b->d_steps, b->d_outs, Q, layerT, dn);
+  HIPT(hipGetLastError());
+  tmark(b, 1);
+  unsigned long long hn = 0;
+  HIPT(hipMemcpyAsync(&hn, dn, sizeof(hn), hipMemcpyDeviceToHost, b->stream));
+  if (read_outs(b, outs)) return -1;
+  HIPT(hipFree(dn));
+  if (b->timing) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+    b->last_ms[0] = ms;
+  }
+  if (n_evals) *n_evals = hn;
+  return 0;
+}
+
 __global__ void k_loop_rng(Inst* I, const amcd_step* S) {
   const Inst& d = I[blockIdx.x];
   LoopState* ls = d.ls;
@@ -1867,15 +2276,20 @@ static int launch_event_loop(amcd_batch* b) {
       hipLaunchKernelGGL((k_event_loop<512, 8>), dim3(n), dim3(512), 0, b->stream, b->d_inst, b->d_steps,
                          b->d_outs);
       break;
-    default:
+    case 3:
       hipLaunchKernelGGL((k_event_loop<512, 16>), dim3(n), dim3(512), 0, b->stream, b->d_inst, b->d_steps,
+                         b->d_outs);
+      break;
+    default:
+      hipLaunchKernelGGL((k_event_loop<512, 0>), dim3(n), dim3(512), 0, b->stream, b->d_inst, b->d_steps,
                          b->d_outs);
       break;
   }
   HIPT(hipGetLastError());
   return 0;
 }
-static const int kTierCap[4] = {192 * 4, 448 * 4, 448 * 8, 448 * 16};
+static const int kNTiers = 5;  // the last one keeps the cells in memory: no limit below the active capacity
+static const int kTierCap[kNTiers] = {192 * 4, 448 * 4, 448 * 8, 448 * 16, 0x7fffffff};
 
 static int read_outs(amcd_batch* b, amcd_out* outs) {
   HIPT(hipMemcpyAsync(b->h_outs, b->d_outs, sizeof(amcd_out) * b->n, hipMemcpyDeviceToHost, b->stream));
@@ -1964,8 +2378,12 @@ static int regrow_tier(amcd_batch* b, amcd_out* outs) {
     bool need = false;
     for (int i = 0; i < b->n; ++i) need = need || outs[i].status == 2;
     if (!need) return 0;
-    if (b->tier >= 3) {
-      g_dev_err = "more growing cells than the largest capture-loop tier holds";
+    if (b->tier >= kNTiers - 1) {
+      int worst = 0;
+      for (int i = 0; i < b->n; ++i)
+        if (outs[i].status == 2 && outs[i].n_slots >= outs[worst].n_slots) worst = i;
+      g_dev_err = "instance " + std::to_string(worst) + ": " + std::to_string(outs[worst].n_slots) +
+                  " growing cells and the memory-resident capture tier still asks for more";
       return -1;
     }
     b->tier++;
@@ -1978,7 +2396,7 @@ static void pick_tier(amcd_batch* b, const amcd_out* outs) {
   int mx = 0;
   for (int i = 0; i < b->n; ++i) mx = std::max(mx, outs[i].n_slots);
   int t = 0;
-  while (t < 3 && kTierCap[t] < mx + mx / 8) ++t;
+  while (t < kNTiers - 1 && kTierCap[t] < mx + mx / 8) ++t;
   b->tier = t;
 }
 
@@ -2054,18 +2472,50 @@ int amcd_remap_grains(amcd_batch* b, int i, const int* remap, int ngrain) {
   return 0;
 }
 
-int amcd_write_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, const int* gtmp) {
+int amcd_init_plate(amcd_batch* b, int i, int nzh, const double* xs, int ng) {
+  if (i < 0 || i >= b->n || ng <= 0 || nzh <= 0) return -1;
   Inst& d = b->h[i];
-  int nl = ilaserLoc - iz1;
-  size_t cnt = (size_t)d.nxy * (size_t)(nl > 0 ? nl : 0);
-  int* dg = nullptr;
-  HIPT(hipMalloc(&dg, sizeof(int) * (cnt + 1)));
-  if (cnt) HIPT(hipMemcpyAsync(dg, gtmp, sizeof(int) * cnt, hipMemcpyHostToDevice, b->stream));
-  hipLaunchKernelGGL(k_write_layer, dim3((d.N + 255) / 256), dim3(256), 0, b->stream, b->d_inst, i, iz1, ilaserLoc,
-                     gbase, dg);
+  if (nzh > d.p.nz) nzh = d.p.nz;
+  double* dx = nullptr;
+  HIPT(hipMalloc(&dx, sizeof(double) * 3 * (size_t)ng));
+  HIPT(hipMemcpyAsync(dx, xs, sizeof(double) * 3 * (size_t)ng, hipMemcpyHostToDevice, b->stream));
+  const long long n = (long long)d.nxy * nzh;
+  hipLaunchKernelGGL(k_voronoi<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, b->d_inst, i, 0,
+                     nzh, dx, ng, nullptr, nullptr);
+  HIPT(hipGetLastError());
   HIPT(hipMemsetAsync(d.zliq, 0xff, (size_t)d.p.nz * 4, b->stream));
   HIPT(hipStreamSynchronize(b->stream));
-  HIPT(hipFree(dg));
+  HIPT(hipFree(dx));
+  return 0;
+}
+
+int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, const double* xs, int ng, int* n_used) {
+  if (i < 0 || i >= b->n || ng <= 0) return -1;
+  Inst& d = b->h[i];
+  const int nl = ilaserLoc - iz1;
+  const size_t cnt = (size_t)d.nxy * (size_t)(nl > 0 ? nl : 0);
+  double* dx = nullptr;
+  int *lab = nullptr, *used = nullptr;
+  HIPT(hipMalloc(&dx, sizeof(double) * 3 * (size_t)ng));
+  HIPT(hipMalloc(&lab, sizeof(int) * (cnt + 1)));
+  HIPT(hipMalloc(&used, sizeof(int) * (2 * (size_t)ng + 1)));  // used[ng], rank[ng], total
+  int* rank = used + ng;
+  int* tot = used + 2 * ng;
+  HIPT(hipMemcpyAsync(dx, xs, sizeof(double) * 3 * (size_t)ng, hipMemcpyHostToDevice, b->stream));
+  HIPT(hipMemsetAsync(used, 0, sizeof(int) * (2 * (size_t)ng + 1), b->stream));
+  if (cnt)
+    hipLaunchKernelGGL(k_voronoi<true>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, b->stream, b->d_inst, i,
+                       iz1, nl, dx, ng, lab, used);
+  hipLaunchKernelGGL(k_site_rank, dim3(1), dim3(1024), 0, b->stream, used, ng, rank, tot);
+  hipLaunchKernelGGL(k_write_layer, dim3((d.N + 255) / 256), dim3(256), 0, b->stream, b->d_inst, i, iz1, ilaserLoc,
+                     gbase, lab, rank);
+  HIPT(hipGetLastError());
+  HIPT(hipMemsetAsync(d.zliq, 0xff, (size_t)d.p.nz * 4, b->stream));
+  HIPT(hipMemcpyAsync(n_used, tot, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  HIPT(hipStreamSynchronize(b->stream));
+  HIPT(hipFree(dx));
+  HIPT(hipFree(lab));
+  HIPT(hipFree(used));
   return 0;
 }
 

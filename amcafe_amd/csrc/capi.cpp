@@ -10,6 +10,7 @@
 
 #include "../../include/amcafe.h"
 #include "amcafe_dev.h"
+#include "h5out.h"
 #include "host_ref.h"
 
 using namespace amcafe;
@@ -55,6 +56,7 @@ static amcd_params make_params(const int* nX, double dx, double tL, double tS, d
   p.Avel = Avel;
   p.nvel = nvel;
   p.seed1 = kSeed1;
+  p.bmV = bmV;
   return p;
 }
 
@@ -229,6 +231,19 @@ extern "C" int amcafe_analytic_temp_curr(amcafe_domain* d, const amcafe_laser* l
   return AMCAFE_OK;
 }
 
+extern "C" int amcafe_easm_temp_curr(amcafe_domain* d, const amcafe_laser* laser, double bmP, double layerT,
+                                     int* all_melt, long long* n_integrals) {
+  if (!d || !laser) return fail(AMCAFE_ERR_ARG, "null argument");
+  amcd_step s = laser_step(laser);
+  std::vector<amcd_out> o(1);
+  unsigned long long ne = 0;
+  if (amcd_easm(d->dev, &s, bmP, layerT, o.data(), &ne)) return dev_fail();
+  d->have_T = true;
+  if (all_melt) *all_melt = o[0].any_melt ? 0 : 1;
+  if (n_integrals) *n_integrals = (long long)ne;
+  return AMCAFE_OK;
+}
+
 extern "C" int amcafe_update_voxels(amcafe_domain* d, int tInd, int ilaserLoc, amcafe_stats* stats) {
   if (!d) return fail(AMCAFE_ERR_ARG, "null domain");
   if (!d->have_T) return fail(AMCAFE_ERR_ARG, "amcafe_analytic_temp_curr must run before amcafe_update_voxels");
@@ -261,6 +276,7 @@ struct Run {
   bool done = false;
   std::vector<int> Build;  // VoxelsCA::Build (host; changes only at output steps)
   std::vector<int> out_steps;
+  std::string out_prefix;  // output files <prefix><tInd>.h5 when set (S/main1.C:53)
   amcd_out last;
 };
 
@@ -297,30 +313,14 @@ static int init_plate(amcafe_sim* s, int i) {
     Xs[3 * j + 1] = xrand(r.bp_generator) * Ly;
     Xs[3 * j + 2] = xrand(r.bp_generator) * r.bp_height;
   }
-  std::vector<int> label;
-  voronoi_plate(g, r.Nzh, Xs, label);
+  if (amcd_init_plate(s->dev, i, r.Nzh, Xs.data(), Ngrain)) return dev_fail();
   Engine g1(531231);
   unsigned int sdloc = unsigned(double(g1()) / double(g1.max()) * pow(2.0, 32.0));
   s->sa.generate(Ngrain, sdloc, r.cTheta);
   r.nGrain = Ngrain;
-  const long long N = g.nvox();
-  const int nxy = g.nX[0] * g.nX[1];
-  std::vector<int> gid((size_t)N, 0), vst((size_t)N, 0);
-  std::vector<double> cen(3 * (size_t)N, 0.0);
-  r.Build.assign((size_t)N, 0);
-  for (size_t j = 0; j < label.size(); ++j) {
-    gid[j] = label[j] + 1;
-    vst[j] = ((long long)j < (long long)(r.Nzh - 1) * nxy) ? 3 : 2;
-  }
-  for (long long j = 0; j < N; ++j) {
-    if (gid[j] == 0) continue;
-    int k = (int)(j / nxy), rr = (int)(j - (long long)k * nxy), jy = rr / g.nX[0], ix = rr - jy * g.nX[0];
-    cen[3 * j] = (double(ix) + .5) * g.dX[0];
-    cen[3 * j + 1] = (double(jy) + .5) * g.dX[1];
-    cen[3 * j + 2] = (double(k) + .5) * g.dX[2];
-    r.Build[j] = 1;
-  }
-  if (amcd_upload(s->dev, i, gid.data(), vst.data(), nullptr, cen.data(), nullptr)) return dev_fail();
+  // InitializeVoxels marks every voxel with a grain as built: the whole plate (S/VoxelsCA.C:157-160)
+  r.Build.assign((size_t)g.nvox(), 0);
+  std::fill(r.Build.begin(), r.Build.begin() + (size_t)r.Nzh * g.nX[0] * g.nX[1], 1);
   return upload_rot(s, i);
 }
 
@@ -343,16 +343,8 @@ static int add_layer(amcafe_sim* s, int i) {
     Xs[3 * j + 1] = xrand(r.genlayer) * Ly;
     Xs[3 * j + 2] = xrand(r.genlayer) * g.layerT + zloc0;
   }
-  std::vector<int> first;
-  std::vector<char> used;
-  voronoi_layer(g, iz1, g.nZlayer, Xs, first, used);
-  std::vector<int> rank(ng2, 0);
-  int i2 = 0;
-  for (int j = 0; j < ng2; ++j)
-    if (used[j]) rank[j] = i2++;
-  std::vector<int> gtmp(first.size());
-  for (size_t j = 0; j < first.size(); ++j) gtmp[j] = rank[first[j]] + 1;
-  if (amcd_write_layer(s->dev, i, iz1, g.ilaserLoc, r.nGrain, gtmp.data())) return dev_fail();
+  int i2 = 0;  // sites that label a voxel of the layer
+  if (amcd_add_layer(s->dev, i, iz1, g.ilaserLoc, r.nGrain, Xs.data(), ng2, &i2)) return dev_fail();
   r.nGrain += i2 + 1;  // one more than the labels used (S/VoxelsCA.C:967-972)
   std::vector<double> aa;
   s->sa.generate(i2 + 1, sdloc, aa);
@@ -380,6 +372,26 @@ static int clean_layer(amcafe_sim* s, int i) {
 
 // VoxelsCA::SetBuild (S/VoxelsCA.C:76-115) as set membership over grain ids: voxels
 // of every grain that has a melted voxel, then of every grain with a Build voxel.
+// VoxelsCA::WriteToHDF1 of instance i's current state
+static int write_run(amcafe_sim* s, int i, const char* path) {
+  Run& r = s->runs[i];
+  const size_t N = (size_t)r.g.nvox();
+  std::vector<int> gid(N), vs(N);
+  std::vector<double> T(N);
+  if (amcd_download(s->dev, i, gid.data(), vs.data(), nullptr, nullptr, nullptr, T.data())) return dev_fail();
+  std::string err;
+  const int nX[3] = {r.g.nX[0], r.g.nX[1], r.g.nX[2]};
+  const double dX[3] = {r.g.dX[0], r.g.dX[1], r.g.dX[2]};
+  if (write_hdf5(path, nX, dX, gid.data(), vs.data(), r.Build.data(), T.data(), r.cTheta.data(), r.nGrain, err))
+    return fail(AMCAFE_ERR_IO, err);
+  return AMCAFE_OK;
+}
+static int write_scheduled(amcafe_sim* s, int i) {
+  const Run& r = s->runs[i];
+  if (r.out_prefix.empty()) return AMCAFE_OK;
+  return write_run(s, i, (r.out_prefix + std::to_string(r.TtInd) + ".h5").c_str());
+}
+
 static int set_build(amcafe_sim* s, int i) {
   Run& r = s->runs[i];
   size_t N = (size_t)r.g.nvox();
@@ -496,10 +508,12 @@ extern "C" int amcafe_sim_step(amcafe_sim* s, int* n_running) {
     if (indOut == 0 || r.done || (g.inewlayerflg == 1 && g.outNL == 0)) {
       r.out_steps.push_back(r.TtInd);
       if ((rc = set_build(s, i))) return rc;
+      if ((rc = write_scheduled(s, i))) return rc;  // S/main1.C:83-84
     }
     g.update_time2(g.bmDelT);
     if (g.inewlayerflg == 1 && (rc = add_layer(s, i))) return rc;
     if (r.done && (rc = clean_layer(s, i))) return rc;  // S/main1.C:94
+    if (r.done && (rc = write_scheduled(s, i))) return rc;  // S/main1.C:95-97
     if (!r.done) running++;
   }
   if (n_running) *n_running = running;
@@ -558,8 +572,30 @@ extern "C" int amcafe_sim_timings(amcafe_sim* s, double* ms, int n) {
 }
 
 extern "C" int amcafe_sim_write_hdf5(amcafe_sim* s, int i, const char* path) {
-  (void)s;
-  (void)i;
-  (void)path;
-  return fail(AMCAFE_ERR_IO, "HDF5 output is not built into this library yet");
+  if (!s || i < 0 || i >= (int)s->runs.size() || !path) return fail(AMCAFE_ERR_ARG, "bad instance or path");
+  return write_run(s, i, path);
+}
+
+extern "C" int amcafe_sim_set_output(amcafe_sim* s, int i, const char* prefix) {
+  if (!s || i < 0 || i >= (int)s->runs.size()) return fail(AMCAFE_ERR_ARG, "bad instance");
+  s->runs[i].out_prefix = prefix ? prefix : "";
+  return AMCAFE_OK;
+}
+
+extern "C" int amcafe_write_hdf5_fields(const char* path, const int* nX, const double* dX, const int* gID,
+                                        const int* vState, const int* Build, const double* TempCurr,
+                                        const double* cTheta, int nGrain) {
+  if (!path || !nX || !dX || !gID || !vState || !Build || !TempCurr || (nGrain > 0 && !cTheta) || nGrain < 0 ||
+      nX[0] <= 0 || nX[1] <= 0 || nX[2] <= 0)
+    return fail(AMCAFE_ERR_ARG, "bad output arguments");
+  std::string err;
+  if (amcafe::write_hdf5(path, nX, dX, gID, vState, Build, TempCurr, cTheta, nGrain, err))
+    return fail(AMCAFE_ERR_IO, err);
+  return AMCAFE_OK;
+}
+
+extern "C" int amcafe_ipf_colours(const double* cTheta, int nGrain, float* rgb9) {
+  if (nGrain < 0 || (nGrain > 0 && (!cTheta || !rgb9))) return fail(AMCAFE_ERR_ARG, "bad grain table");
+  amcafe::ipf_colours(cTheta, nGrain, rgb9);
+  return AMCAFE_OK;
 }

@@ -4,14 +4,11 @@
 #include "host_ref.h"
 
 #include <algorithm>
-#include <atomic>
-#include <functional>
 #include <cmath>
 #include <limits>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
-#include <thread>
 
 namespace amcafe {
 
@@ -356,81 +353,6 @@ void rot_table(const std::vector<double>& cTheta, int first, int count, std::vec
     double ax[3] = {c[1], c[2], c[3]};
     rot_from_axis_angle(c[0], ax, &rot9[9 * (size_t)g]);
   }
-}
-
-// ---------------------------------------------------------------------------------
-static void parallel_for(long long n, const std::function<void(long long, long long)>& f) {
-  unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < 4096 || nt == 1) {
-    f(0, n);
-    return;
-  }
-  std::vector<std::thread> th;
-  long long chunk = (n + nt - 1) / nt;
-  for (unsigned t = 0; t < nt; ++t) {
-    long long a = t * chunk, b = std::min(n, a + chunk);
-    if (a >= b) break;
-    th.emplace_back(f, a, b);
-  }
-  for (auto& t : th) t.join();
-}
-
-void voronoi_plate(const Deck& g, int Nzh, const std::vector<double>& Xs, std::vector<int>& label) {
-  const int nx = g.nX[0], nxy = g.nX[0] * g.nX[1];
-  const long long n = (long long)Nzh * nxy;
-  const int ng = (int)(Xs.size() / 3);
-  label.assign((size_t)n, 0);
-  parallel_for(n, [&](long long a, long long b) {
-    for (long long j = a; j < b; ++j) {
-      int j3 = (int)(j / nxy), j2 = (int)((j - (long long)nxy * j3) / nx), j1 = (int)(j - (long long)nxy * j3 - (long long)nx * j2);
-      double zloc = (j3 + .5) * g.dX[2], yloc = (j2 + .5) * g.dX[1], xloc = (j1 + .5) * g.dX[0];
-      int best = 0;
-      double bd = 0;
-      for (int q = 0; q < ng; ++q) {  // S/BasePlate.C:110-113, first minimum
-        double ax = xloc - Xs[3 * q], ay = yloc - Xs[3 * q + 1], az = zloc - Xs[3 * q + 2];
-        double d = ax * ax + ay * ay + az * az;
-        if (q == 0 || d < bd) {
-          bd = d;
-          best = q;
-        }
-      }
-      label[(size_t)j] = best;
-    }
-  });
-}
-
-void voronoi_layer(const Deck& g, int iz1, int nlayer, const std::vector<double>& Xs, std::vector<int>& first,
-                   std::vector<char>& used) {
-  const int nx = g.nX[0], ny = g.nX[1];
-  const long long nxy = (long long)nx * ny, n = nxy * nlayer;
-  const int ng = (int)(Xs.size() / 3);
-  first.assign((size_t)n, 0);
-  used.assign(ng, 0);
-  std::vector<std::vector<char>> usedT(16, std::vector<char>(ng, 0));
-  std::atomic<int> next_slot(0);
-  parallel_for(n, [&](long long a, long long b) {
-    int slot = next_slot.fetch_add(1);
-    std::vector<char>& u = usedT[slot % 16];
-    for (long long j = a; j < b; ++j) {
-      int j3a = iz1 + (int)(j / nxy);
-      long long r = j % nxy;
-      int j2a = (int)(r / nx), j1a = (int)(r % nx);
-      double dsqc = 1e6;
-      int i2 = 0;
-      for (int jz = 0; jz < ng; ++jz) {  // S/VoxelsCA.C:925-930
-        double ax = j1a * g.dX[0] - Xs[3 * jz], ay = j2a * g.dX[1] - Xs[3 * jz + 1], az = j3a * g.dX[2] - Xs[3 * jz + 2];
-        double dsq = ax * ax + ay * ay + az * az;
-        if (dsq < dsqc) {
-          i2 = jz;
-          dsqc = dsq;
-        }
-      }
-      first[(size_t)j] = i2;
-      u[i2] = 1;
-    }
-  });
-  for (auto& u : usedT)
-    for (int q = 0; q < ng; ++q) used[q] |= u[q];
 }
 
 }  // namespace amcafe

@@ -66,10 +66,4 @@ struct Orientation {
 void rot_from_axis_angle(double omega, const double* ax, double* R9);
 void rot_table(const std::vector<double>& cTheta, int first, int count, std::vector<double>& rot9);
 
-// brute-force Voronoi labels, exact reference arithmetic, split over host threads.
-// plate: voxel centres (S/BasePlate.C:103-114); layer: voxel corners (S/VoxelsCA.C:922-936)
-void voronoi_plate(const Deck& g, int Nzh, const std::vector<double>& sites, std::vector<int>& label);
-void voronoi_layer(const Deck& g, int iz1, int nlayer, const std::vector<double>& sites, std::vector<int>& first,
-                   std::vector<char>& used);
-
 }  // namespace amcafe

@@ -79,6 +79,14 @@ class Sim:
         check(lib().amcafe_sim_get_grains(self.h, i, ct.ctypes.data, ctypes.byref(n)))
         return ct
 
+    def write_hdf5(self, path, i=0):
+        """WriteToHDF1 of instance i's current state to the file `path`."""
+        check(lib().amcafe_sim_write_hdf5(self.h, i, path.encode()))
+
+    def set_output(self, prefix, i=0):
+        """The reference's output schedule: <prefix><tInd>.h5 at every output step (None: off)."""
+        check(lib().amcafe_sim_set_output(self.h, i, None if prefix is None else prefix.encode()))
+
     def set_timing(self, on=True):
         check(lib().amcafe_sim_set_timing(self.h, 1 if on else 0))
 
@@ -140,6 +148,13 @@ class Domain:
         m = ctypes.c_int()
         check(lib().amcafe_analytic_temp_curr(self.h, ctypes.byref(las), ctypes.byref(m)))
         return m.value
+
+    def easm_temperature(self, x, y, z, dirp, gth, ilaserLoc, bmP, layerT):
+        """TempField::EASM on the device; returns (all voxels at T >= tL, source integrals evaluated)."""
+        las = Laser(x, y, z, dirp, gth, ilaserLoc)
+        m, n = ctypes.c_int(), ctypes.c_longlong()
+        check(lib().amcafe_easm_temp_curr(self.h, ctypes.byref(las), bmP, layerT, ctypes.byref(m), ctypes.byref(n)))
+        return m.value, n.value
 
     def update_voxels(self, tInd, ilaserLoc):
         s = Stats()

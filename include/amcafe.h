@@ -92,6 +92,17 @@ int amcafe_domain_get_grains(amcafe_domain* d, double* cTheta, int* nGrain);
 /* TempField::AnalyticTempCurr: TempCurr and BuildjID on the device; *any_melt
  * is the end-of-scan input "some voxel at or above tL" (TempField.C:202-205). */
 int amcafe_analytic_temp_curr(amcafe_domain* d, const amcafe_laser* laser, int* any_melt);
+/* TempField::EASM (S/TempField.C:79-137; Utilities.C:60-140, 228-249) in place of
+ * AnalyticTempCurr: the Gaussian moving source (absorptivity 0.56, radius 50 um, power bmP)
+ * integrated over the last 8e-4 s with the 25-point Gauss-Legendre rule, 316L properties by a
+ * damped Picard iteration (tolerance 10 K, at most 100 iterations), FP64 on the device.  0 K
+ * above the laser plane, 0.99 T0 ahead of the laser in the top layer (layerT).  *all_melt: every
+ * voxel at T >= tL (the end-of-scan test of S/TempField.C:117-126); *n_integrals: source
+ * integrals evaluated.  Either may be NULL.  The reference cannot select this model from a deck
+ * (its EASM keyword never matches, S/Grid.C:198-199,276); UpdateVoxels after it reports
+ * AMCAFE_ERR_DEVICE, since growth velocities exist only for the analytic temperatures. */
+int amcafe_easm_temp_curr(amcafe_domain* d, const amcafe_laser* laser, double bmP, double layerT,
+                          int* all_melt, long long* n_integrals);
 /* VoxelsCA::UpdateVoxels on the current TempCurr; tInd = Grid::tInd. */
 int amcafe_update_voxels(amcafe_domain* d, int tInd, int ilaserLoc, amcafe_stats* stats);
 
@@ -123,8 +134,26 @@ int amcafe_sim_stats(amcafe_sim* s, int inst, amcafe_stats* stats);
 int amcafe_sim_get_fields(amcafe_sim* s, int inst, int* gID, int* vState, double* TempCurr, double* extents,
                           double* centroidOct, int* BuildjID, int* Build);
 int amcafe_sim_get_grains(amcafe_sim* s, int inst, double* cTheta, int* nGrain);
-/* WriteToHDF1 layout: Step0/{dims,VoxelDX,gID,vState,Build,Temperature,IPFz,IPFx,IPFy,angleAxis} */
+/* VoxelsCA::WriteToHDF1 (S/VoxelsCA.C:1592-1856) of one instance's current state to the file
+ * `path` (the reference names it CA3D<deck stem><tInd>.h5, S/main1.C:53,80-84).  Layout:
+ * group Step0 with dims int32[3], VoxelDX float32[3], gID, vState, Build int32[N],
+ * Temperature float32[N], IPFz, IPFx, IPFy float32[3N], angleAxis float32[4 nGrain]; root
+ * attribute NumSteps = 1.  Needs the HDF5 (>= 1.10) shared library at run time, loaded with
+ * dlopen from $AMCAFE_HDF5_LIB or libhdf5.so.103; AMCAFE_ERR_IO when it is absent. */
 int amcafe_sim_write_hdf5(amcafe_sim* s, int inst, const char* path);
+/* The reference driver's own output schedule (S/main1.C:76-84, 95-97): from now on every
+ * output step (tInd % outint == 0, the last step, or a new layer with outNL 0) writes
+ * <prefix><tInd>.h5, and the end of the run writes it again after the final CleanLayer.
+ * prefix NULL stops writing. */
+int amcafe_sim_set_output(amcafe_sim* s, int inst, const char* prefix);
+/* WriteToHDF1 on host buffers in the reference layouts (no device involved): nX, dX of the
+ * grid, gID / vState / Build int32[N], TempCurr double[N], cTheta double[4 nGrain]. */
+int amcafe_write_hdf5_fields(const char* path, const int* nX, const double* dX, const int* gID,
+                             const int* vState, const int* Build, const double* TempCurr,
+                             const double* cTheta, int nGrain);
+/* The IPF colours WriteToHDF1 gives a grain (S/VoxelsCA.C:1597-1817): 9 floats per grain,
+ * RGB of the build direction (IPFz), of x (IPFx) and of y (IPFy). */
+int amcafe_ipf_colours(const double* cTheta, int nGrain, float* rgb9);
 /* per-stage device time of the last step in ms when enabled: temperature, solid->mushy,
  * active list, growing-cell setup, capture loop, scatter + mushy->solid */
 int amcafe_sim_set_timing(amcafe_sim* s, int on);

@@ -389,6 +389,18 @@ struct Sim {
   int nlayerTot = 0;
   bool done = false;
   std::vector<int> out_steps;
+  // what WriteToHDF1 would write at each output (S/main1.C:83-84, 95-97), when recording
+  struct Snapshot {
+    int tInd;
+    std::vector<int> gID, vState, Build;
+    std::vector<double> T, cTheta;
+  };
+  bool record_outputs = false;
+  std::vector<Snapshot> snaps;
+  void snapshot() {
+    if (!record_outputs) return;
+    snaps.push_back(Snapshot{TtInd, gID, vState, Build, TempCurr, cTheta});
+  }
   StepStats st;
   // Moore offsets in ascending linear-index order (S/Grid.C:399-414 + sort at :375)
   int noff = 0;
@@ -550,6 +562,98 @@ struct Sim {
         g.isp = 0;
         g.indlayer += 1;
       }
+    }
+  }
+
+  // ---- TempField::EASM (S/TempField.C:79-137) with Utilities::EASM_Temp_LG, Integral,
+  // Temp_Fun_G and Materials_316L (S/Utilities.C:60-140, 228-249).  The reference's rule is
+  // GSL's fixed 25-point Gauss-Legendre (not in this image); gl_rule() computes the same
+  // rule independently (Newton on P25 in quad precision, rounded to double).
+  static void mat316(double T, double* M) {  // M = {cP, k, rho, alpha}
+    double rho, cP, k;
+    if (T < 1660) rho = 8084.2 - 0.42086 * T - 3.8942 * pow(10, -5) * T * T;
+    else if (T >= 1660 && T < 3090) rho = 7432.7 + 3.9338 * pow(10, -2) * T - 1.8007 * pow(10, -4) * T * T;
+    else rho = 6124.72;
+    if (T < 1660) cP = 458.984 + 0.1328 * T;
+    else cP = 769.855;
+    if (T < 1660) k = 9.248 + 1.571 * pow(10, -2) * T;
+    else if (T >= 1660 && T < 3090) k = 12.41 + 3.279 * pow(10, -3) * T;
+    else k = 21.6465;
+    M[0] = cP; M[1] = k; M[2] = rho; M[3] = k / (rho * cP);
+  }
+  double easm_integral(const double* L, double T, const double* R, long* nint) const {
+    const double t = 80.0e-5, to = 0.0, sigma = 50.0e-6, eta = 0.56, Q = g.bmP;
+    double M[4];
+    mat316(T, M);
+    const double C = eta * Q / (M_PI * M[2] * M[0] * sqrt(4 * M_PI * M[3]));
+    const double x0 = L[0] - g.bmV * t;
+    const double shft = (to + t) / 2.0, slp = (t - to) / 2.0;
+    double sum = 0.0;
+    for (int i = 0; i < 25; ++i) {
+      const double x = shft + slp * gl_x[i], w = gl_w[i] * slp;
+      const double Lx = x0 + g.bmV * x, Ly = L[1], Lz = L[2];
+      const double f = C * pow(t - x, -0.5) / (2 * M[3] * (t - x) + pow(sigma, 2)) *
+                       exp(-1 * (pow(R[0] - Lx, 2) + pow(R[1] - Ly, 2)) / (4 * M[3] * (t - x) + 2 * pow(sigma, 2)) -
+                           pow(std::fabs(R[2] - Lz), 2) / (4 * M[3] * (t - x)));
+      sum += w * f;
+    }
+    ++*nint;
+    return 300 + sum;
+  }
+  double easm_temp_lg(const double* L, const double* R, double T, long* nint) const {
+    double Told = T;
+    int n = 0, nlim = 100;
+    double tol = 10;
+    double Tnew = easm_integral(L, T, R, nint);
+    while (std::fabs(Told - Tnew) > tol && n < nlim) {
+      Told = Tnew;
+      double T_temp = easm_integral(L, Told, R, nint);
+      Tnew = 0.5 * (Told + T_temp);
+      n++;
+    }
+    return Tnew;
+  }
+  // the voxel loop with an explicit laser (x, y, z, cos gth, sin gth); returns all_of(T >= tL)
+  int easm(double x, double y, double z, double cg, double sg, long* nint) {
+    for (int j = 0; j < N; ++j) {
+      TempCurr[j] = 0;
+      int j3 = j / nXY, j2 = (j - nXY * j3) / g.nX[0], j1 = j - nXY * j3 - g.nX[0] * j2;
+      double x0 = (double(j1) + .5) * (g.dX[0]);
+      double y0 = (double(j2) + .5) * (g.dX[1]);
+      double zp = (double(j3) + .5) * (g.dX[2]);
+      if (zp > z) continue;
+      double xp = cg * (x0 - g.LX[0] / 2.) + sg * (y0 - g.LX[1] / 2.) + g.LX[0] / 2.;
+      double yp = -sg * (x0 - g.LX[0] / 2.) + cg * (y0 - g.LX[1] / 2.) + g.LX[1] / 2.;
+      const double R[3] = {xp, yp, zp}, L[3] = {x, y, z};
+      if ((x0 > x) && (zp > (z - g.layerT))) TempCurr[j] = .99 * g.T0;
+      else TempCurr[j] = easm_temp_lg(L, R, TempCurr[j], nint);
+      if (TempCurr[j] > g.tL) BuildjID[j] = 1;
+      else if (TempCurr[j] < g.T0) BuildjID[j] = 0;
+    }
+    for (int j = 0; j < N; ++j)
+      if (!(TempCurr[j] >= g.tL)) return 0;
+    return 1;
+  }
+  double gl_x[25], gl_w[25];
+  void gl_rule() {
+    const int n = 25;
+    const __float128 pi = 4 * atanq((__float128)1), tiny = (__float128)1e-33;
+    for (int i = 0; i < n; ++i) {
+      __float128 x = cosq(pi * (i + (__float128)0.75) / (n + (__float128)0.5)), p0, p1, dp;
+      for (int it = 0; it < 100; ++it) {
+        p0 = 1; p1 = x;
+        for (int k = 2; k <= n; ++k) { __float128 p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k; p0 = p1; p1 = p2; }
+        dp = n * (x * p1 - p0) / (x * x - 1);
+        __float128 dx = p1 / dp;
+        x -= dx;
+        if (fabsq(dx) < tiny) break;
+      }
+      p0 = 1; p1 = x;
+      for (int k = 2; k <= n; ++k) { __float128 p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k; p0 = p1; p1 = p2; }
+      dp = n * (x * p1 - p0) / (x * x - 1);
+      if (fabsq(x) < tiny) x = 0;  // the middle root of the odd P25 is exactly 0
+      gl_x[n - 1 - i] = (double)x;  // roots come out descending
+      gl_w[n - 1 - i] = (double)(2 / ((1 - x * x) * dp * dp));
     }
   }
 
@@ -940,10 +1044,14 @@ struct Sim {
     if (indOut == 0 || done || (g.inewlayerflg == 1 && g.outNL == 0)) {
       out_steps.push_back(TtInd);
       set_build();
+      snapshot();
     }
     g.update_time2(DelT);
     if (g.inewlayerflg == 1) add_layer1();
-    if (done) clean_layer();  // S/main1.C:94 after the loop
+    if (done) {
+      clean_layer();  // S/main1.C:94 after the loop
+      snapshot();     // S/main1.C:95-97
+    }
     return done ? 1 : 0;
   }
 };
@@ -988,6 +1096,36 @@ int orc_step(void* h) {
   return s->finish_step();
 }
 int orc_done(void* h) { return ((orc::Sim*)h)->done ? 1 : 0; }
+// TempField::EASM on the oracle's grid with an explicit laser; returns all_of(T >= tL)
+int orc_easm(void* h, double x, double y, double z, double cg, double sg, long* n_integrals) {
+  orc::Sim* s = (orc::Sim*)h;
+  s->gl_rule();
+  long n = 0;
+  int r = s->easm(x, y, z, cg, sg, &n);
+  if (n_integrals) *n_integrals = n;
+  return r;
+}
+// the oracle's 25-point Gauss-Legendre rule on [-1, 1]
+void orc_gl25(double* x, double* w) {
+  orc::Sim s;
+  s.gl_rule();
+  for (int i = 0; i < 25; ++i) { x[i] = s.gl_x[i]; w[i] = s.gl_w[i]; }
+}
+// output recording: snapshots of gID, vState, Build, TempCurr and cTheta at every output
+void orc_record_outputs(void* h, int on) { ((orc::Sim*)h)->record_outputs = on != 0; }
+int orc_n_outputs(void* h) { return (int)((orc::Sim*)h)->snaps.size(); }
+// k-th snapshot: returns its tInd and sets *ngrain; buffers may be null
+int orc_output(void* h, int k, int* gid, int* vs, int* build, double* T, double* ct, int* ngrain) {
+  const orc::Sim::Snapshot& o = ((orc::Sim*)h)->snaps[k];
+  const size_t n = o.gID.size();
+  if (gid) std::memcpy(gid, o.gID.data(), n * 4);
+  if (vs) std::memcpy(vs, o.vState.data(), n * 4);
+  if (build) std::memcpy(build, o.Build.data(), n * 4);
+  if (T) std::memcpy(T, o.T.data(), n * 8);
+  if (ct) std::memcpy(ct, o.cTheta.data(), o.cTheta.size() * 8);
+  if (ngrain) *ngrain = (int)(o.cTheta.size() / 4);
+  return o.tInd;
+}
 void orc_dims(void* h, int* d) {
   orc::Sim* s = (orc::Sim*)h;
   d[0] = s->g.nX[0]; d[1] = s->g.nX[1]; d[2] = s->g.nX[2];

@@ -1,5 +1,6 @@
 """ctypes access to the test-only oracle (oracle/_build/liboracle.so)."""
 import ctypes
+import math
 import os
 import shutil
 import tempfile
@@ -36,6 +37,14 @@ def _load(path):
     L.orc_events.argtypes = [_P, _P, _P, _P, ctypes.c_long]
     L.orc_events.restype = ctypes.c_long
     L.orc_laser_inputs.argtypes = [_P, _P]
+    L.orc_easm.argtypes = [_P] + [ctypes.c_double] * 5 + [ctypes.POINTER(ctypes.c_long)]
+    L.orc_easm.restype = _I
+    L.orc_gl25.argtypes = [_P, _P]
+    L.orc_record_outputs.argtypes = [_P, _I]
+    L.orc_n_outputs.argtypes = [_P]
+    L.orc_n_outputs.restype = _I
+    L.orc_output.argtypes = [_P, _I] + [_P] * 5 + [ctypes.POINTER(_I)]
+    L.orc_output.restype = _I
     L.orc_generate_samples.argtypes = [_I, ctypes.c_uint, _P]
     L.orc_uniform_stream.argtypes = [ctypes.c_ulong, _I, _P]
     L.orc_rot_mat.argtypes = [ctypes.c_double, _P, _P]
@@ -145,6 +154,32 @@ class Oracle:
         o = np.zeros(6)
         self.L.orc_laser_inputs(self.h, o.ctypes.data)
         return o[0], o[1], o[2], o[3], o[4], int(o[5])
+
+    def easm(self, x, y, z, gth):
+        """TempField::EASM on the oracle's grid (TempCurr, BuildjID) for a laser at (x, y, z)
+        and grid rotation gth; returns (all_of(T >= tL), source integrals evaluated)."""
+        n = ctypes.c_long()
+        r = self.L.orc_easm(self.h, x, y, z, math.cos(gth), math.sin(gth), ctypes.byref(n))
+        return r, n.value
+
+    def record_outputs(self, on=True):
+        self.L.orc_record_outputs(self.h, 1 if on else 0)
+
+    def outputs(self):
+        """What WriteToHDF1 writes at each output of the run so far: a list of
+        (tInd, dict(gID, vState, Build, TempCurr, cTheta)), later entries win per tInd."""
+        n = self.L.orc_n_outputs(self.h)
+        N = self.N
+        res = []
+        for k in range(n):
+            ng = ctypes.c_int()
+            self.L.orc_output(self.h, k, None, None, None, None, None, ctypes.byref(ng))
+            o = dict(gID=np.zeros(N, np.int32), vState=np.zeros(N, np.int32), Build=np.zeros(N, np.int32),
+                     TempCurr=np.zeros(N), cTheta=np.zeros(4 * ng.value))
+            t = self.L.orc_output(self.h, k, o["gID"].ctypes.data, o["vState"].ctypes.data, o["Build"].ctypes.data,
+                                  o["TempCurr"].ctypes.data, o["cTheta"].ctypes.data, ctypes.byref(ng))
+            res.append((t, o))
+        return res
 
     def done(self):
         return self.L.orc_done(self.h)

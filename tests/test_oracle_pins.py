@@ -297,3 +297,49 @@ def test_bare_oracle_takes_over_a_running_state():
         for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
             assert np.array_equal(fa[k], fb[k], equal_nan=fa[k].dtype.kind == "f"), k
         assert np.array_equal(a.grains(), b.grains())
+
+
+# ---------------------------------------------------------------------------------------
+# EASM temperature (S/TempField.C:79-137, S/Utilities.C:60-140, 228-249)
+def _gl25_table():
+    import re
+    text = open(os.path.join(ROOT, "amcafe_amd", "csrc", "gl25.h")).read()
+    vals = [float.fromhex(v) for v in re.findall(r"-?0x[0-9a-fp.+-]+", text)]
+    return np.array(vals[:25]), np.array(vals[25:])
+
+
+def test_gauss_legendre_rule_of_easm():
+    """The 25-point rule the device compiles in (gl25.h, mpmath at 60 digits) equals the
+    oracle's own (Newton on P25 in quad precision) bit for bit, and integrates every
+    monomial of degree <= 49 on [-1, 1] exactly (to rounding)."""
+    x, w = np.zeros(25), np.zeros(25)
+    oracle_lib(False).orc_gl25(x.ctypes.data, w.ctypes.data)
+    tx, tw = _gl25_table()
+    assert np.array_equal(x, tx) and np.array_equal(w, tw)
+    assert np.all(np.diff(x) > 0) and np.array_equal(x, -x[::-1]) and np.array_equal(w, w[::-1])
+    for k in range(50):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(math.fsum(w * x ** k) - exact) < 1e-14, k
+
+
+def test_easm_regions_and_far_field():
+    """Closed forms of TempField::EASM on the small deck's grid: 0 K above the laser plane,
+    0.99 T0 in the top layer ahead of the laser, and with the source 2 mm away the
+    ambient 300 K (the Picard iteration stops after two integrals); with the source on the
+    grid the melt pool exceeds tL under the beam and marks BuildjID."""
+    o = Oracle(read("t_small.in"), crmath=False)
+    dx, (nx, ny, nz), il = 2.5e-6, (40, 24, 20), 12
+    z = il * dx
+    allm, nint = o.easm(-2e-3, ny * dx / 2, z, 0.0)
+    T = o.fields()["TempCurr"].reshape(nz, ny, nx)
+    top = int(round((z - 20e-6) / dx))          # planes with zp > z - layerT
+    assert allm == 0
+    assert np.all(T[il:] == 0.0)
+    assert np.all(T[top:il] == 0.99 * 300.0)
+    assert np.all((T[:top] >= 300.0) & (T[:top] < 300.0 + 1e-9))
+    assert nint == 2 * top * nx * ny
+    allm, nint = o.easm(nx * dx / 2, ny * dx / 2, z, 0.0)
+    f = o.fields()
+    T = f["TempCurr"].reshape(nz, ny, nx)
+    assert T[il - 1, ny // 2, nx // 2 - 1] > 1733.0
+    assert np.all(f["BuildjID"][f["TempCurr"] > 1733.0] == 1)
