@@ -786,12 +786,17 @@ __device__ static inline void recentre(const Inst& d, int vcap, int vnew, const 
   locX[1] = R[1] * dnx[0] + R[4] * dnx[1] + R[7] * dnx[2];
   locX[2] = R[2] * dnx[0] + R[5] * dnx[1] + R[8] * dnx[2];
   double dr = fabs(locX[0]) + fabs(locX[1]) + fabs(locX[2]);
-  // octahedron vertices +-dr e_i; the octant's three vertices (S/VoxelsCA.h:61-76)
-  const int sInd[8][3] = {{0, 1, 2}, {1, 2, 3}, {0, 2, 4}, {2, 3, 4}, {0, 1, 5}, {1, 3, 5}, {0, 4, 5}, {3, 4, 5}};
-  int jInd = 4 * (signbit(locX[2]) ? 1 : 0) + 2 * (signbit(locX[1]) ? 1 : 0) + (signbit(locX[0]) ? 1 : 0);
+  // octahedron vertices +-dr e_i (vertex c is +dr e_c, vertex c+3 is -dr e_c); the octant's
+  // three vertices in ascending vertex order, the reference's table of S/VoxelsCA.h:61-76
+  // ({0,1,2},{1,2,3},{0,2,4},{2,3,4},{0,1,5},{1,3,5},{0,4,5},{3,4,5} by octant), computed
+  // instead of looked up so that no memory access sits in the chain
+  const int a0 = signbit(locX[0]) ? 3 : 0, a1 = signbit(locX[1]) ? 4 : 1, a2 = signbit(locX[2]) ? 5 : 2;
+  const int vlo = min(min(a0, a1), a2), vhi = max(max(a0, a1), a2);
+  const int vsel[3] = {vlo, a0 + a1 + a2 - vlo - vhi, vhi};
   double V[3][3], S0[3][3];
+#pragma unroll
   for (int q = 0; q < 3; ++q) {
-    int vi = sInd[jInd][q];
+    int vi = vsel[q];
     for (int c = 0; c < 3; ++c) {
       double sgn = (vi % 3 == c) ? (vi < 3 ? 1.0 : -1.0) : 0.0;
       S0[q][c] = sgn;
@@ -1271,7 +1276,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
 // memory and every event streams the lane's cells (slots me, me + NO, ...).  Same
 // decisions, same arithmetic, same order of additions as the register tiers.
 template <int NT>
-__device__ static void owner_loop_mem(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
+__device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
                                       long long& visits, long long& evals) {
   constexpr int NO = NT - 64;
   constexpr int NW = NT / 64;
@@ -1297,14 +1302,25 @@ __device__ static void owner_loop_mem(const Inst& d, EngineShared& sh, const uns
   x.i1 = sh.r_i1;
   x.dt = 0.0;
   bool apply_first = go == 2;
+  // the lane's cells are streamed MB at a time: all loads of a batch are issued before
+  // any of them is used
+  constexpr int MB = 8;
   while (go > 0) {
     if (!apply_first) {
       // ---- EVAL: the wave's smallest upper bound, then the cells whose lower bound reaches it
       float u = fInf, lmin = fInf;
-      for (int s = me; s < ns; s += NO) {
-        const float2 lh = d.gLH[s];
-        u = fminf(u, lh.y);
-        lmin = fminf(lmin, lh.x);
+      for (int s0 = me; s0 < ns; s0 += MB * NO) {
+        float2 lh[MB];
+#pragma unroll
+        for (int q = 0; q < MB; ++q) {
+          const int s = s0 + q * NO;
+          lh[q] = s < ns ? d.gLH[s] : make_float2(fInf, fInf);
+        }
+#pragma unroll
+        for (int q = 0; q < MB; ++q) {
+          u = fminf(u, lh[q].y);
+          lmin = fminf(lmin, lh[q].x);
+        }
       }
       for (int m = 32; m >= 1; m >>= 1) u = fminf(u, __shfl_xor(u, m));
       Best b = best_none();
@@ -1380,32 +1396,48 @@ __device__ static void owner_loop_mem(const Inst& d, EngineShared& sh, const uns
     ++nev;
     const bool all = nev >= (1 << 20);
     if (all) nev = 0;
-    for (int s = me; s < ns; s += NO) {
-      int2 hd = d.gHd[s];
-      if (hd.y < 0) continue;
-      float2 lh = d.gLH[s];
-      double e = d.gExt[s];
-      const double v = d.gVhat[s];
-      if (!x.nuc) {  // growth (S/VoxelsCA.C:511-516)
-        const double g = e + v * Dt;
-        if (g < 0.0) {
-          e = 0.0;
-          lh.x = -fInf;
-        } else {
-          e = g;
-        }
-        d.gExt[s] = e;
+    for (int s0 = me; s0 < ns; s0 += MB * NO) {
+      int2 hdb[MB];
+      float2 lhb[MB];
+      double eb[MB], vb[MB];
+#pragma unroll
+      for (int q = 0; q < MB; ++q) {
+        const int s = s0 + q * NO;
+        const bool in = s < ns;
+        hdb[q] = in ? d.gHd[s] : make_int2(0, -1);
+        lhb[q] = in ? d.gLH[s] : make_float2(fInf, fInf);
+        eb[q] = in ? d.gExt[s] : 0.0;
+        vb[q] = in ? d.gVhat[s] : 0.0;
       }
-      if (hd.y == i1) {
-        const SlotHead h = slot_head(d, dead, s, hd.x - s * AMCD_NB + 1, i1);
-        hd = make_int2(h.c1, h.tgt1);
-        lh = make_float2(fInf, fInf);
-        if (h.tgt1 >= 0) tau_bounds(h.D1, e, v, T, lh.x, lh.y);
-        d.gHd[s] = hd;
-        d.gLH[s] = lh;
-      } else if (lh.x == -fInf || all) {
-        tau_bounds(d.cD[hd.x], e, v, T, lh.x, lh.y);
-        d.gLH[s] = lh;
+#pragma unroll
+      for (int q = 0; q < MB; ++q) {
+        const int s = s0 + q * NO;
+        int2 hd = hdb[q];
+        if (hd.y < 0) continue;
+        float2 lh = lhb[q];
+        double e = eb[q];
+        const double v = vb[q];
+        if (!x.nuc) {  // growth (S/VoxelsCA.C:511-516)
+          const double g = e + v * Dt;
+          if (g < 0.0) {
+            e = 0.0;
+            lh.x = -fInf;
+          } else {
+            e = g;
+          }
+          d.gExt[s] = e;
+        }
+        if (hd.y == i1) {
+          const SlotHead h = slot_head(d, dead, s, hd.x - s * AMCD_NB + 1, i1);
+          hd = make_int2(h.c1, h.tgt1);
+          lh = make_float2(fInf, fInf);
+          if (h.tgt1 >= 0) tau_bounds(h.D1, e, v, T, lh.x, lh.y);
+          d.gHd[s] = hd;
+          d.gLH[s] = lh;
+        } else if (lh.x == -fInf || all) {
+          tau_bounds(d.cD[hd.x], e, v, T, lh.x, lh.y);
+          d.gLH[s] = lh;
+        }
       }
     }
     skip = i1;
@@ -1422,12 +1454,22 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   __shared__ EngineShared sh;
   __shared__ int s_go;
   __shared__ unsigned s_dead[AMCD_DEAD_BITS / 32];
-  const Inst& d = I[blockIdx.x];
+  // The descriptor and step record are copied to LDS: every event ends in a barrier, after
+  // which the compiler must re-read anything in global memory, and each array access would
+  // otherwise wait for its base pointer to come back from global memory first.
+  __shared__ Inst s_inst;
+  __shared__ amcd_step s_step;
+  if (I[blockIdx.x].ls->done) return;
+  if (threadIdx.x == 0) {
+    s_inst = I[blockIdx.x];
+    s_step = S[blockIdx.x];
+  }
+  __syncthreads();
+  const Inst& d = s_inst;
   LoopState* ls = d.ls;
-  if (ls->done) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const amcd_params& p = d.p;
-  const amcd_step& st = S[blockIdx.x];
+  const amcd_step& st = s_step;
   const int Na = ls->Na;
   if (Na <= AMCD_DEAD_BITS) {  // liveness bitmap from the current states
     for (int wd = tid; wd < (Na + 31) / 32; wd += NT) {
@@ -2390,13 +2432,13 @@ static int regrow_tier(amcd_batch* b, amcd_out* outs) {
     if (launch_event_loop(b) || read_outs(b, outs)) return -1;
   }
 }
-// smallest tier holding 1.125x the largest growing list of the step just done (a step
-// that outgrows it resumes in the next tier)
+// smallest tier holding 1/32 more than the largest growing list of the step just done (a
+// step that outgrows it resumes in the next tier)
 static void pick_tier(amcd_batch* b, const amcd_out* outs) {
   int mx = 0;
   for (int i = 0; i < b->n; ++i) mx = std::max(mx, outs[i].n_slots);
   int t = 0;
-  while (t < kNTiers - 1 && kTierCap[t] < mx + mx / 8) ++t;
+  while (t < kNTiers - 1 && kTierCap[t] < mx + mx / 32) ++t;
   b->tier = t;
 }
 

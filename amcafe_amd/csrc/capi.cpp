@@ -85,16 +85,21 @@ struct Pool {
 
 // Run UpdateVoxels (optionally with the fused temperature) for a batch, refilling
 // the nucleation pools of instances that used them up.
+// Pool size for an instance whose previous step nucleated `last` grains: twice that plus a
+// margin, so that a refill (another launch of the capture loop) is rare.
+static int pool_size(int last) { return kPoolChunk + 2 * (last > 0 ? last : 0); }
+
 static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<amcd_out>& out,
-                      std::vector<Pool>& pools, const Orientation& sa, bool fused) {
+                      std::vector<Pool>& pools, const Orientation& sa, bool fused, const std::vector<int>& last_nucs) {
   const int n = (int)st.size();
   for (int i = 0; i < n; ++i) {
     if (st[i].skip) continue;
+    const int np = pool_size(last_nucs[i]);
     pools[i].clear();
-    pools[i].build(sa, st[i].tInd, st[i].nGrain, kPoolChunk);
-    st[i].n_pool = kPoolChunk;
-    if (amcd_reserve_grains(dev, i, st[i].nGrain + 4 * kPoolChunk + 64)) return dev_fail();
-    if (amcd_set_pool(dev, i, pools[i].rot.data(), kPoolChunk)) return dev_fail();
+    pools[i].build(sa, st[i].tInd, st[i].nGrain, np);
+    st[i].n_pool = np;
+    if (amcd_reserve_grains(dev, i, st[i].nGrain + 4 * np + 64)) return dev_fail();
+    if (amcd_set_pool(dev, i, pools[i].rot.data(), np)) return dev_fail();
   }
   int rc = fused ? amcd_step_fused(dev, st.data(), out.data()) : amcd_update_voxels(dev, st.data(), out.data());
   if (rc) return dev_fail();
@@ -109,7 +114,7 @@ static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<a
       }
       if (out[i].status != 1) continue;
       pend = true;
-      int nn = st[i].n_pool + kPoolChunk;
+      int nn = 2 * st[i].n_pool;
       pools[i].build(sa, st[i].tInd, st[i].nGrain, nn);
       st[i].n_pool = nn;
       if (amcd_reserve_grains(dev, i, st[i].nGrain + nn + 64)) return dev_fail();
@@ -148,6 +153,7 @@ struct amcafe_domain {
   int nGrain = 0;
   Orientation sa;
   bool have_T = false;
+  int last_nucs = 0;  // nucleations of the previous step (sizes the orientation pool)
 };
 
 extern "C" int amcafe_domain_create(const amcafe_domain_desc* d, amcafe_domain** out) {
@@ -254,12 +260,13 @@ extern "C" int amcafe_update_voxels(amcafe_domain* d, int tInd, int ilaserLoc, a
   st[0].nGrain = d->nGrain;
   std::vector<amcd_out> out(1);
   std::vector<Pool> pools(1);
-  int rc = run_update(d->dev, st, out, pools, d->sa, false);
+  int rc = run_update(d->dev, st, out, pools, d->sa, false, std::vector<int>(1, d->last_nucs));
   if (rc) return rc;
   d->have_T = false;
   int used = out[0].pool_used;
   d->cTheta.insert(d->cTheta.end(), pools[0].ct.begin(), pools[0].ct.begin() + 4 * (size_t)used);
   d->nGrain = out[0].nGrain;
+  d->last_nucs = out[0].n_nucleations;
   if (stats) to_stats(out[0], stats);
   return AMCAFE_OK;
 }
@@ -277,7 +284,7 @@ struct Run {
   std::vector<int> Build;  // VoxelsCA::Build (host; changes only at output steps)
   std::vector<int> out_steps;
   std::string out_prefix;  // output files <prefix><tInd>.h5 when set (S/main1.C:53)
-  amcd_out last;
+  amcd_out last = {};
 };
 
 struct amcafe_sim {
@@ -488,7 +495,9 @@ extern "C" int amcafe_sim_step(amcafe_sim* s, int* n_running) {
     if (n_running) *n_running = 0;
     return AMCAFE_OK;
   }
-  int rc = run_update(s->dev, st, out, pools, s->sa, true);
+  std::vector<int> last(n, 0);
+  for (int i = 0; i < n; ++i) last[i] = s->runs[i].last.n_nucleations;
+  int rc = run_update(s->dev, st, out, pools, s->sa, true, last);
   if (rc) return rc;
   running = 0;
   for (int i = 0; i < n; ++i) {

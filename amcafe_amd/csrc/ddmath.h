@@ -249,7 +249,7 @@ __host__ __device__ static inline double amc_shape_factor_ref(double dx, double 
   ph < pi4 ? ph = pi2 - ph : ph;
   return sqrt(amc_cos_q1(th) * amc_sin_q1(ph));
 }
-__host__ __device__ static inline double amc_shape_factor(double dx, double dy, double dz) {
+__host__ __device__ static inline __attribute__((always_inline)) double amc_shape_factor(double dx, double dy, double dz) {
   const double pi4 = 0x1.921fb54442d18p-1, pi2 = 0x1.921fb54442d18p+0;  // M_PI/4.0, M_PI/2.0
   const double ax = fabs(dx), ay = fabs(dy), az = fabs(dz);
   const double rho = sqrt(dx * dx + dy * dy);
