@@ -51,6 +51,7 @@ __device__ unsigned long long g_phase[16];
 
 #define ZB 1024     // voxels per compaction block (one per thread)
 #define GB 256      // threads per grid-wide block
+#define SB 4096     // voxels per solid->mushy block (GB threads, 16 voxels each)
 
 // -------------------------------------------------------------------------------------
 // device structures
@@ -68,11 +69,15 @@ struct LoopState {
   int pend_tgt;      // active index of the voxel to nucleate
   int Na, nmushy0;
   int zhot0, zhot1;  // z planes the last temperature evaluation could have melted
+  // T holds the analytic field of a laser plane at lz_prev (tvalid = 1): every voxel is xi
+  // where its hot flag (vs bit 7) is set, else T0 above lz_prev and tS below it
+  int tvalid, pad1;
+  double lz_prev;
 };
 
 struct Inst {
   amcd_params p;
-  int N, nxy, nblkZ, nblkG;
+  int N, nxy, nblkZ, nblkG, nblkS, nblkT;
   // grid fields
   uint8_t* vs;
   int* gid;
@@ -82,7 +87,8 @@ struct Inst {
   uint8_t* bj;
   int* amap;
   unsigned* zliq;  // per z plane: bit 0 = had liquid at the last active-set pass (or unknown)
-  int *bcA, *bcM;  // per-ZB-block counts -> exclusive offsets
+  int *bcA, *bcM;  // per-ZB-block counts -> exclusive offsets within chunks of 1024 blocks
+  int *chA, *chM;  // per chunk: totals -> exclusive offsets
   // active list
   int acap;
   int* aVox;
@@ -167,58 +173,118 @@ __device__ static inline void decode(const Inst& d, int v, int& i, int& j, int& 
 
 // -------------------------------------------------------------------------------------
 // TempField::AnalyticTempCurr, S/TempField.C:155-197, with the optional SetLiquid3.
+// T takes one of three values, T0 (above the laser plane), xi = tL(1+eps) (inside the melt
+// pool) or tS.  Two shortcuts keep the result bit-identical while touching little memory:
+//  * a voxel whose |offset| along some semi-axis exceeds that semi-axis by a relative 2^-40
+//    is outside the ellipsoid without the divisions (RN is monotone, so the rounded
+//    quotient is >= 1 and so is the rounded sum of squares);
+//  * while tvalid holds, the previous T of a voxel is known from its hot flag and the
+//    previous laser plane, so T, the flag byte and BuildjID are stored only where they change.
+#define TV 16  // voxels per thread (one 16-byte load of their state bytes)
 __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, int ninst, const amcd_step* S,
                                                     amcd_out* O, int setliquid) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
   const Inst& d = I[ii];
   const amcd_step& st = S[ii];
   if (st.skip) return;
-  int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
-  bool hot = false;
-  if (v == 0) {  // planes with (k+.5)dz in (lz - max depth, lz] are the only ones T can reach tL in
-    const double depth = fmax(d.p.a1[2], d.p.a1[5]);
-    d.ls->zhot0 = max(0, (int)floor((st.lz - depth) / d.p.dz - 0.5) - 1);
-    d.ls->zhot1 = min(d.p.nz - 1, (int)floor(st.lz / d.p.dz - 0.5) + 1);
+  const int v0 = ((blockIdx.x - bstart[ii]) * GB + threadIdx.x) * TV;
+  const amcd_params& p = d.p;
+  if (v0 == 0) {  // planes with (k+.5)dz in (lz - max depth, lz] are the only ones T can reach tL in
+    const double depth = fmax(p.a1[2], p.a1[5]);
+    d.ls->zhot0 = max(0, (int)floor((st.lz - depth) / p.dz - 0.5) - 1);
+    d.ls->zhot1 = min(p.nz - 1, (int)floor(st.lz / p.dz - 0.5) + 1);
   }
-  if (v < d.N) {
+  bool anyhot = false;
+  if (v0 < d.N) {
+    const bool valid = d.ls->tvalid != 0 && p.tS < p.tL && p.T0 < p.tL;
+    const double lzp = d.ls->lz_prev;
+    const double e1 = 1 + 0x1p-40;
+    const double lim1 = p.a1[1] * e1, lim2 = p.a1[2] * e1, lim3 = p.a1[3] * e1, lim4 = p.a1[4] * e1,
+                 lim5 = p.a1[5] * e1;
+    uint8_t sv[TV];
+    if (v0 + TV <= d.N) {
+      const uint4 w = *reinterpret_cast<const uint4*>(d.vs + v0);
+      const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int q = 0; q < TV; ++q) sv[q] = (uint8_t)(ws[q >> 2] >> (8 * (q & 3)));
+    } else {
+#pragma unroll
+      for (int q = 0; q < TV; ++q) sv[q] = v0 + q < d.N ? d.vs[v0 + q] : 0;
+    }
+    // planes outside (lz - max depth, lz] (with a plane of margin) cannot reach tL: the
+    // ellipsoid tests need |zp - lz| < depth, so T is T0 or tS there without geometry
+    const double depth = fmax(p.a1[2], p.a1[5]);
+    const int kh0 = (int)floor((st.lz - depth) / p.dz - 0.5) - 1, kh1 = (int)floor(st.lz / p.dz - 0.5) + 1;
     int i, j, k;
-    decode(d, v, i, j, k);
-    const amcd_params& p = d.p;
-    double x0 = ((double)i + .5) * p.dx;
-    double y0 = ((double)j + .5) * p.dy;
-    double zp = ((double)k + .5) * p.dz;
-    double T = p.T0;
-    if (!(zp > st.lz)) {
-      double xp = st.cg * (x0 - p.hx) + st.sg * (y0 - p.hy) + p.hx;
-      double yp = (-st.sg) * (x0 - p.hx) + st.cg * (y0 - p.hy) + p.hy;
-      double r0 = xp - st.lx, r1 = yp - st.ly, r2 = zp - st.lz;
-      if (st.dirp * r0 > 0) {
-        double q1 = r1 / p.a1[1], q2 = r2 / p.a1[2];
-        double dsq = q1 * q1 + q2 * q2;
-        T = (dsq < 1.0 && (fabs(r0) < p.bmDX0)) ? p.xi : p.tS;
-      } else {
-        double q0 = r0 / p.a1[3], q1 = r1 / p.a1[4], q2 = r2 / p.a1[5];
-        double dsq = q0 * q0 + q1 * q1 + q2 * q2;
-        T = (dsq < 1.0) ? p.xi : p.tS;
+    decode(d, v0, i, j, k);
+#pragma unroll
+    for (int q = 0; q < TV; ++q) {
+      const int v = v0 + q;
+      if (v >= d.N) break;
+      const double x0 = ((double)i + .5) * p.dx;
+      const double y0 = ((double)j + .5) * p.dy;
+      const double zp = ((double)k + .5) * p.dz;
+      double T = p.T0;
+      if (k < kh0 || k > kh1) {
+        if (!(zp > st.lz)) T = p.tS;
+      } else if (!(zp > st.lz)) {
+        const double xp = st.cg * (x0 - p.hx) + st.sg * (y0 - p.hy) + p.hx;
+        const double yp = (-st.sg) * (x0 - p.hx) + st.cg * (y0 - p.hy) + p.hy;
+        const double r0 = xp - st.lx, r1 = yp - st.ly, r2 = zp - st.lz;
+        T = p.tS;
+        if (st.dirp * r0 > 0) {
+          if (fabs(r0) < p.bmDX0 && !(fabs(r1) > lim1) && !(fabs(r2) > lim2)) {
+            const double q1 = r1 / p.a1[1], q2 = r2 / p.a1[2];
+            const double dsq = q1 * q1 + q2 * q2;
+            if (dsq < 1.0) T = p.xi;
+          }
+        } else if (!(fabs(r0) > lim3) && !(fabs(r1) > lim4) && !(fabs(r2) > lim5)) {
+          const double q0 = r0 / p.a1[3], q1 = r1 / p.a1[4], q2 = r2 / p.a1[5];
+          const double dsq = q0 * q0 + q1 * q1 + q2 * q2;
+          if (dsq < 1.0) T = p.xi;
+        }
       }
-      if (T > p.tL) d.bj[v] = 1;
-      else if (T < p.T0) d.bj[v] = 0;
-    }
-    d.T[v] = T;
-    hot = T >= p.tL;
-    uint8_t s = d.vs[v] & 3;
-    if (setliquid && hot) {
-      if (v < d.nxy) {
-        s = 2;
-      } else if (v < st.ilaserLoc * d.nxy) {
-        s = 1;
-        d.gid[v] = 0;
+      const uint8_t s0 = sv[q];
+      const bool hot = T >= p.tL;
+      const double Told = (s0 & 0x80) ? p.xi : (zp > lzp ? p.T0 : p.tS);
+      if (!valid || T != Told) d.T[v] = T;
+      if (!(zp > st.lz)) {
+        if (T > p.tL) {
+          if (d.bj[v] != 1) d.bj[v] = 1;
+        } else if (T < p.T0) {
+          d.bj[v] = 0;
+        }
       }
-      d.ext[v] = 0.0;
+      uint8_t s = s0 & 3;
+      if (setliquid && hot) {
+        if (v < d.nxy) {
+          s = 2;
+        } else if (v < st.ilaserLoc * d.nxy) {
+          s = 1;
+          d.gid[v] = 0;
+        }
+        d.ext[v] = 0.0;
+      }
+      const uint8_t s1 = s | (s0 & 0x40) | (hot ? 0x80 : 0);
+      if (s1 != s0) d.vs[v] = s1;
+      anyhot = anyhot || hot;
+      if (++i == p.nx) {
+        i = 0;
+        if (++j == p.ny) {
+          j = 0;
+          ++k;
+        }
+      }
     }
-    d.vs[v] = s | (hot ? 0x80 : 0);
   }
-  if (__any(hot) && (threadIdx.x & 63) == 0) atomicOr(&O[ii].any_melt, 1);
+  if (__any(anyhot) && (threadIdx.x & 63) == 0) atomicOr(&O[ii].any_melt, 1);
+}
+// After a temperature pass: the field now matches this step's laser plane.
+__global__ void k_temp_commit(Inst* I, const amcd_step* S, int valid) {
+  const Inst& d = I[blockIdx.x];
+  if (threadIdx.x != 0 || S[blockIdx.x].skip) return;
+  d.ls->tvalid = valid;
+  d.ls->lz_prev = S[blockIdx.x].lz;
 }
 
 // -------------------------------------------------------------------------------------
@@ -359,35 +425,52 @@ __global__ __launch_bounds__(GB) void k_solid_to_mushy(Inst* I, const int* bstar
   int ii = find_inst(bstart, ninst, blockIdx.x);
   const Inst& d = I[ii];
   if (S[ii].skip) return;
-  int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
-  if (v >= d.N) return;
-  int i, j, k;
-  decode(d, v, i, j, k);
   const int nx = d.p.nx, ny = d.p.ny, nz = d.p.nz;
+  const int b0 = (blockIdx.x - bstart[ii]) * SB;
+  const int bend = min(d.N, b0 + SB);
   // liquid exists only in planes that held it at the last active-set pass or that the
-  // last temperature evaluation could melt; other planes' solid voxels cannot change
+  // last temperature evaluation could melt: a block without such planes leaves at once
   const int h0 = d.ls->zhot0, h1 = d.ls->zhot1;
-  bool may = false;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = k + dz;
-    if (z >= 0 && z < nz) may = may || (d.zliq[z] & 1u) || (z >= h0 && z <= h1);
+  const int zb0 = b0 / d.nxy, zb1 = (bend - 1) / d.nxy;
+  bool any = false;
+  for (int z = zb0; z <= zb1; ++z) any = any || (d.zliq[z] & 1u) || (z >= h0 && z <= h1);
+  if (!any) return;
+  // each thread reads 16 consecutive state bytes at once and marks the solid Moore
+  // neighbours of the liquid ones mushy (the same voxels as asking every solid voxel
+  // whether a neighbour is liquid; concurrent marks store the same byte)
+  constexpr int PT = SB / GB;
+  const int v0 = b0 + (int)threadIdx.x * PT;
+  if (v0 >= bend) return;
+  uint8_t st[PT];
+  if (v0 + PT <= d.N) {
+    const uint4 w = *reinterpret_cast<const uint4*>(d.vs + v0);
+    const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (int q = 0; q < PT; ++q) st[q] = (uint8_t)(ws[q >> 2] >> (8 * (q & 3)));
+  } else {
+#pragma unroll
+    for (int q = 0; q < PT; ++q) st[q] = v0 + q < d.N ? d.vs[v0 + q] : 0;
   }
-  if (!may) return;
-  uint8_t s = d.vs[v];
-  if ((s & 3) != 3) return;
-  for (int dz = -1; dz <= 1; ++dz) {
-    int z = k + dz;
-    if (z < 0 || z >= nz) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      int y = j + dy;
-      if (y < 0 || y >= ny) continue;
-      const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * nx;
-      for (int dx = -1; dx <= 1; ++dx) {
-        int x = i + dx;
-        if (x < 0 || x >= nx || (dx == 0 && dy == 0 && dz == 0)) continue;
-        if ((row[x] & 3) == 1) {
-          d.vs[v] = (s & ~3) | 2;
-          return;
+  bool liq_any = false;
+#pragma unroll
+  for (int q = 0; q < PT; ++q) liq_any = liq_any || (st[q] & 3) == 1;
+  if (!liq_any) return;
+  for (int q = 0; q < PT; ++q) {
+    if ((st[q] & 3) != 1) continue;
+    int i, j, k;
+    decode(d, v0 + q, i, j, k);
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = k + dz;
+      if (z < 0 || z >= nz) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = j + dy;
+        if (y < 0 || y >= ny) continue;
+        uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * nx;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = i + dx;
+          if (x < 0 || x >= nx) continue;
+          const uint8_t sn = row[x];
+          if ((sn & 3) == 3) row[x] = (sn & ~3) | 2;
         }
       }
     }
@@ -439,17 +522,27 @@ __global__ __launch_bounds__(ZB) void k_active_flags(Inst* I, const int* bstart,
         }
       }
     }
-    d.vs[v] = (s & ~0x40) | (act ? 0x40 : 0);
+    const uint8_t s1 = (s & ~0x40) | (act ? 0x40 : 0);
+    if (s1 != s) d.vs[v] = s1;
   }
-  // planes holding liquid now (for the next step's solid->mushy pass): one atomic per run
-  // of liquid lanes in a plane
+  // planes holding liquid now (for the next step's solid->mushy pass): at most one atomic
+  // per wave and plane, and none once the flag is seen set (every liquid run of a plane
+  // would otherwise hit the same word)
   {
     const bool liq = v < d.N && (d.vs[v] & 3) == 1;
     const int kz = v < d.N ? v / d.nxy : 0;
     const int lane = threadIdx.x & 63;
-    const int kprev = __shfl_up(kz, 1);
-    const bool lprev = __shfl_up((int)liq, 1) != 0;
-    if (liq && (lane == 0 || !lprev || kprev != kz)) atomicOr(&d.zliq[kz], 1u);
+    bool first;
+    if (d.nxy >= 64) {  // a wave spans at most two planes
+      const unsigned long long bl = __ballot(liq), same = __ballot(kz == __shfl(kz, 0));
+      const unsigned long long g1 = bl & same, g2 = bl & ~same;
+      first = (g1 && lane == __ffsll((long long)g1) - 1) || (g2 && lane == __ffsll((long long)g2) - 1);
+    } else {
+      const int kprev = __shfl_up(kz, 1);
+      const bool lprev = __shfl_up((int)liq, 1) != 0;
+      first = liq && (lane == 0 || !lprev || kprev != kz);
+    }
+    if (first && !(d.zliq[kz] & 1u)) atomicOr(&d.zliq[kz], 1u);
   }
   unsigned long long ba = __ballot(act), bm = __ballot(mushy);
   int w = threadIdx.x >> 6;
@@ -469,28 +562,58 @@ __global__ __launch_bounds__(ZB) void k_active_flags(Inst* I, const int* bstart,
   }
 }
 
-// Exclusive scan of the block counts, one workgroup per instance; sizes the step.
+// Exclusive scan of the per-block counts in two levels: chunks of 1024 blocks scanned
+// by one workgroup each (offsets within the chunk, chunk totals), then one workgroup
+// per instance over the chunk totals.
+__global__ __launch_bounds__(1024) void k_scan_chunks(Inst* I, const int* cstart, int ninst) {
+  __shared__ int wa[16], wm[16];
+  const int ii = find_inst(cstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  const int c = blockIdx.x - cstart[ii];
+  const int b = c * 1024 + (int)threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int a = 0, m = 0;
+  if (b < d.nblkZ) {
+    a = d.bcA[b];
+    m = d.bcM[b];
+  }
+  int ia = a, im = m;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int ta = __shfl_up(ia, o), tm = __shfl_up(im, o);
+    if (lane >= o) {
+      ia += ta;
+      im += tm;
+    }
+  }
+  if (lane == 63) {
+    wa[w] = ia;
+    wm[w] = im;
+  }
+  __syncthreads();
+  int pa = 0, pm = 0;
+  for (int q = 0; q < w; ++q) {
+    pa += wa[q];
+    pm += wm[q];
+  }
+  if (b < d.nblkZ) {
+    d.bcA[b] = pa + ia - a;
+    d.bcM[b] = pm + im - m;
+  }
+  if (threadIdx.x == 1023) {
+    d.chA[c] = pa + ia;
+    d.chM[c] = pm + im;
+  }
+}
 __global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd_out* O) {
   __shared__ int sA[1024], sM[1024];
   const Inst& d = I[blockIdx.x];
-  int nb = d.nblkZ, t = threadIdx.x;
-  int per = (nb + 1023) / 1024;
-  int b0 = t * per, b1 = min(nb, b0 + per);
+  const int nc = (d.nblkZ + 1023) / 1024, t = threadIdx.x;
+  const int per = (nc + 1023) / 1024;
+  const int c0 = t * per, c1 = min(nc, c0 + per);
   int a = 0, m = 0;
-  // each thread's run of block counts is read 16 at a time with independent loads, so a
-  // chunk costs one memory latency instead of sixteen
-  for (int b = b0; b < b1; b += 16) {
-    int va[16], vm[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      va[q] = (b + q < b1) ? d.bcA[b + q] : 0;
-      vm[q] = (b + q < b1) ? d.bcM[b + q] : 0;
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      a += va[q];
-      m += vm[q];
-    }
+  for (int c = c0; c < c1; ++c) {
+    a += d.chA[c];
+    m += d.chM[c];
   }
   sA[t] = a;
   sM[t] = m;
@@ -503,22 +626,12 @@ __global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd
     __syncthreads();
   }
   int ra = sA[t] - a, rm = sM[t] - m;
-  for (int b = b0; b < b1; b += 16) {
-    int va[16], vm[16];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      va[q] = (b + q < b1) ? d.bcA[b + q] : 0;
-      vm[q] = (b + q < b1) ? d.bcM[b + q] : 0;
-    }
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      if (b + q < b1) {
-        d.bcA[b + q] = ra;
-        d.bcM[b + q] = rm;
-      }
-      ra += va[q];
-      rm += vm[q];
-    }
+  for (int c = c0; c < c1; ++c) {
+    const int va = d.chA[c], vm = d.chM[c];
+    d.chA[c] = ra;
+    d.chM[c] = rm;
+    ra += va;
+    rm += vm;
   }
   if (t == 1023) {
     int Na = sA[1023], Nm = sM[1023];
@@ -573,7 +686,7 @@ __global__ __launch_bounds__(ZB) void k_compact(Inst* I, const int* bstart, int 
   }
   __syncthreads();
   if (!act) return;
-  int offA = d.bcA[blk], offM = d.bcM[blk];
+  int offA = d.bcA[blk] + d.chA[blk >> 10], offM = d.bcM[blk] + d.chM[blk >> 10];
   for (int q = 0; q < w; ++q) {
     offA += sA[q];
     offM += sM[q];
@@ -708,46 +821,54 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
   for (int q = 0; q < 3; ++q) d.gC[3 * (size_t)slot + q] = c[q];
   d.gV[slot] = d.aVox[a];
   d.gG[slot] = G;
-  const int* nb = d.aNbr + (size_t)a * AMCD_NB;
-  int n = 0;
-  size_t cb = (size_t)slot * AMCD_NB;
-  for (int q = 0; q < AMCD_NB; ++q) {
-    int t = nb[q];
-    if (t < 0 || d.aVS[t] != 1) continue;
-    double D, f;
-    candidate(d, c, Rl, d.aVox[t], &D, &f);
-    d.cD[cb + n] = D;
-    d.cF[cb + n] = f;
-    d.cR[cb + n] = .5 * (d.ruq[a] + d.ruq[t]);
-    d.cTgt[cb + n] = t;
-    d.cV[cb + n] = d.aVox[t];
-    d.cK[cb + n] = (uint8_t)q;
-    ++n;
+}
+
+// Candidate lists of the growing cells (S/VoxelsCA.C:396-440): one wave per slot, lane q
+// evaluates neighbour q when it is liquid, and the entries are placed by rank in
+// (D, ruqs, scan key) order -- the order the capture loop reads the list in.
+__device__ static inline void grow_list(const Inst& d, int slot, int lane) {
+  const int a = d.gAct[slot];
+  const int t = lane < AMCD_NB ? d.aNbr[(size_t)a * AMCD_NB + lane] : -1;
+  const bool liq = t >= 0 && d.aVS[t] == 1;
+  double D = __builtin_inf(), f = 0.0, r = __builtin_inf();
+  int key = 0x7fffffff, tv = -1;
+  if (liq) {
+    double c[3], R[9];
+    for (int q = 0; q < 3; ++q) c[q] = d.gC[3 * (size_t)slot + q];
+    for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)slot + q];
+    tv = d.aVox[t];
+    candidate(d, c, R, tv, &D, &f);
+    r = d.aRN[(size_t)a * AMCD_NB + lane];
+    key = d.gScan[slot] * 32 + lane;
   }
-  // sort the list by (D, ruqs, key): the capture loop reads it front to back
-  const int kb = d.aScan[a] * 32;
-  for (int x = 1; x < n; ++x) {
-    double D = d.cD[cb + x], f = d.cF[cb + x], r = d.cR[cb + x];
-    int t = d.cTgt[cb + x], tv = d.cV[cb + x];
-    uint8_t kk = d.cK[cb + x];
-    int y = x - 1;
-    while (y >= 0 && cand_less(D, r, kb + kk, d.cD[cb + y], d.cR[cb + y], kb + d.cK[cb + y])) {
-      d.cD[cb + y + 1] = d.cD[cb + y];
-      d.cF[cb + y + 1] = d.cF[cb + y];
-      d.cR[cb + y + 1] = d.cR[cb + y];
-      d.cTgt[cb + y + 1] = d.cTgt[cb + y];
-      d.cV[cb + y + 1] = d.cV[cb + y];
-      d.cK[cb + y + 1] = d.cK[cb + y];
-      --y;
-    }
-    d.cD[cb + y + 1] = D;
-    d.cF[cb + y + 1] = f;
-    d.cR[cb + y + 1] = r;
-    d.cTgt[cb + y + 1] = t;
-    d.cV[cb + y + 1] = tv;
-    d.cK[cb + y + 1] = kk;
+  const unsigned long long bal = __ballot(liq);
+  int rank = 0;
+  for (int j = 0; j < AMCD_NB; ++j) {
+    const double Dj = __shfl(D, j), rj = __shfl(r, j);
+    const int kj = __shfl(key, j);
+    if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
   }
-  d.gN[slot] = (uint8_t)n;
+  const size_t cb = (size_t)slot * AMCD_NB;
+  if (liq) {
+    d.cD[cb + rank] = D;
+    d.cF[cb + rank] = f;
+    d.cR[cb + rank] = r;
+    d.cTgt[cb + rank] = t;
+    d.cV[cb + rank] = tv;
+    d.cK[cb + rank] = (uint8_t)lane;
+  }
+  if (lane == 0) d.gN[slot] = (uint8_t)__popcll(bal);
+}
+// every wave of an instance's blocks takes slots wave, wave + (its waves), ...
+__global__ __launch_bounds__(GB) void k_grow_lists(Inst* I, const int* bstart, int ninst) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (d.ls->done) return;
+  const int lane = threadIdx.x & 63;
+  const int nw = (bstart[ii + 1] - bstart[ii]) * (GB / 64);
+  const int ns = d.ls->nslots;
+  for (int slot = ((blockIdx.x - bstart[ii]) * GB + (int)threadIdx.x) >> 6; slot < ns; slot += nw)
+    if (!d.gDead[slot]) grow_list(d, slot, lane);
 }
 
 // Re-centring of the decentred octahedron after a capture, S/VoxelsCA.C:528-605 and
@@ -1939,8 +2060,9 @@ struct amcd_batch {
   hipStream_t stream = nullptr;
   std::vector<Inst> h;      // host mirror of the descriptors
   Inst* d_inst = nullptr;
-  int *d_bstartG = nullptr, *d_bstartZ = nullptr, *d_astart = nullptr;
-  std::vector<int> bstartG, bstartZ, astart;
+  int *d_bstartG = nullptr, *d_bstartZ = nullptr, *d_astart = nullptr, *d_bstartS = nullptr, *d_bstartT = nullptr,
+      *d_lstart = nullptr, *d_cstart = nullptr;
+  std::vector<int> bstartG, bstartZ, astart, bstartS, bstartT, lstart, cstart;
   amcd_step* d_steps = nullptr;
   amcd_out* d_outs = nullptr;
   amcd_step* h_steps = nullptr;  // pinned
@@ -2022,6 +2144,10 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   }
   for (int q = 0; q < 16; ++q) hipEventCreate(&b->ev[q]);
   b->bstartG.assign(n + 1, 0);
+  b->bstartS.assign(n + 1, 0);
+  b->bstartT.assign(n + 1, 0);
+  b->lstart.assign(n + 1, 0);
+  b->cstart.assign(n + 1, 0);
   b->bstartZ.assign(n + 1, 0);
   b->astart.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) {
@@ -2038,10 +2164,13 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     d.nxy = p[i].nx * p[i].ny;
     d.nblkZ = (d.N + ZB - 1) / ZB;
     d.nblkG = (d.N + GB - 1) / GB;
+    d.nblkS = (d.N + SB - 1) / SB;
+    d.nblkT = (d.N + GB * TV - 1) / (GB * TV);
     size_t Nn = (size_t)d.N;
     if (dalloc(b, &d.vs, Nn) || dalloc(b, &d.gid, Nn) || dalloc(b, &d.ext, Nn) || dalloc(b, &d.cx, Nn) ||
         dalloc(b, &d.cy, Nn) || dalloc(b, &d.cz, Nn) || dalloc(b, &d.T, Nn) || dalloc(b, &d.bj, Nn) ||
         dalloc(b, &d.amap, Nn) || dalloc(b, &d.bcA, (size_t)d.nblkZ) || dalloc(b, &d.bcM, (size_t)d.nblkZ) ||
+        dalloc(b, &d.chA, (size_t)d.nblkZ / 1024 + 1) || dalloc(b, &d.chM, (size_t)d.nblkZ / 1024 + 1) ||
         dalloc(b, &d.zliq, (size_t)p[i].nz) || dalloc(b, &d.ls, 1)) {
       snprintf(err, errlen, "%s", g_dev_err.c_str());
       amcd_destroy(b);
@@ -2069,9 +2198,14 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     d.poolcap = 0;
     d.pool = nullptr;
     b->bstartG[i + 1] = b->bstartG[i] + d.nblkG;
+    b->bstartS[i + 1] = b->bstartS[i] + d.nblkS;
+    b->bstartT[i + 1] = b->bstartT[i] + d.nblkT;
+    b->cstart[i + 1] = b->cstart[i] + (d.nblkZ + 1023) / 1024;
     b->bstartZ[i + 1] = b->bstartZ[i] + d.nblkZ;
   }
   if (dalloc(b, &b->d_inst, (size_t)n) || dalloc(b, &b->d_bstartG, (size_t)n + 1) ||
+      dalloc(b, &b->d_bstartS, (size_t)n + 1) || dalloc(b, &b->d_bstartT, (size_t)n + 1) ||
+      dalloc(b, &b->d_lstart, (size_t)n + 1) || dalloc(b, &b->d_cstart, (size_t)n + 1) ||
       dalloc(b, &b->d_bstartZ, (size_t)n + 1) || dalloc(b, &b->d_astart, (size_t)n + 1) ||
       dalloc(b, &b->d_steps, (size_t)n) || dalloc(b, &b->d_outs, (size_t)n)) {
     snprintf(err, errlen, "%s", g_dev_err.c_str());
@@ -2085,6 +2219,9 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     return -1;
   }
   hipMemcpyAsync(b->d_bstartG, b->bstartG.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
+  hipMemcpyAsync(b->d_bstartS, b->bstartS.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
+  hipMemcpyAsync(b->d_bstartT, b->bstartT.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
+  hipMemcpyAsync(b->d_cstart, b->cstart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
   hipMemcpyAsync(b->d_bstartZ, b->bstartZ.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
   if (push_inst(b) || hipStreamSynchronize(b->stream) != hipSuccess) {
     snprintf(err, errlen, "%s", g_dev_err.c_str());
@@ -2120,6 +2257,7 @@ int amcd_upload(amcd_batch* b, int i, const int* gid, const int* vstate, const d
     for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(vstate[v] & 3);
     HIPT(hipMemcpy(d.vs, tmp.data(), N, hipMemcpyHostToDevice));
     HIPT(hipMemset(d.zliq, 0xff, (size_t)d.p.nz * 4));
+    HIPT(hipMemset(&d.ls->tvalid, 0, sizeof(int)));
   }
   if (bj) {
     for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(bj[v] ? 1 : 0);
@@ -2231,10 +2369,16 @@ static void tmark(amcd_batch* b, int q) {
 }
 
 static int run_temperature(amcd_batch* b, int setliquid) {
-  int nb = b->bstartG[b->n];
-  hipLaunchKernelGGL(k_temperature, dim3(nb), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, b->n, b->d_steps,
+  int nb = b->bstartT[b->n];
+  hipLaunchKernelGGL(k_temperature, dim3(nb), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartT, b->n, b->d_steps,
                      b->d_outs, setliquid);
+  hipLaunchKernelGGL(k_temp_commit, dim3(b->n), dim3(64), 0, b->stream, b->d_inst, b->d_steps, 1);
   HIPT(hipGetLastError());
+  return 0;
+}
+// the device T no longer follows the analytic layout (upload, new layer, EASM)
+static int invalidate_T(amcd_batch* b, int i) {
+  HIPT(hipMemsetAsync(&b->h[i].ls->tvalid, 0, sizeof(int), b->stream));
   return 0;
 }
 
@@ -2255,6 +2399,7 @@ int amcd_easm(amcd_batch* b, const amcd_step* steps, double Q, double layerT, am
   tmark(b, 0);
   hipLaunchKernelGGL(k_easm, dim3(b->bstartG[b->n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, b->n,
                      b->d_steps, b->d_outs, Q, layerT, dn);
+  hipLaunchKernelGGL(k_temp_commit, dim3(b->n), dim3(64), 0, b->stream, b->d_inst, b->d_steps, 0);
   HIPT(hipGetLastError());
   tmark(b, 1);
   unsigned long long hn = 0;
@@ -2289,12 +2434,13 @@ static int launch_update(amcd_batch* b, bool fused_temperature) {
                        b->d_steps);
   }
   tmark(b, 1);
-  hipLaunchKernelGGL(k_solid_to_mushy, dim3(b->bstartG[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartG, n,
+  hipLaunchKernelGGL(k_solid_to_mushy, dim3(b->bstartS[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_bstartS, n,
                      b->d_steps);
   tmark(b, 2);
   hipLaunchKernelGGL(k_zliq_clear, dim3(n), dim3(256), 0, b->stream, b->d_inst, b->d_steps);
   hipLaunchKernelGGL(k_active_flags, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
                      b->d_steps);
+  hipLaunchKernelGGL(k_scan_chunks, dim3(b->cstart[n]), dim3(1024), 0, b->stream, b->d_inst, b->d_cstart, n);
   hipLaunchKernelGGL(k_scan, dim3(n), dim3(1024), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
   hipLaunchKernelGGL(k_compact, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
                      b->d_steps);
@@ -2350,6 +2496,10 @@ static int launch_active(amcd_batch* b) {
   HIPT(hipMemcpyAsync(b->d_astart, b->astart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
                      b->d_outs);
+  b->lstart[0] = 0;
+  for (int i = 0; i < n; ++i) b->lstart[i + 1] = b->lstart[i] + std::min(256, (b->h[i].acap + 3) / 4);
+  HIPT(hipMemcpyAsync(b->d_lstart, b->lstart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(k_grow_lists, dim3(b->lstart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_lstart, n);
   tmark(b, 4);
   if (launch_event_loop(b)) return -1;
   tmark(b, 5);
@@ -2526,6 +2676,7 @@ int amcd_init_plate(amcd_batch* b, int i, int nzh, const double* xs, int ng) {
                      nzh, dx, ng, nullptr, nullptr);
   HIPT(hipGetLastError());
   HIPT(hipMemsetAsync(d.zliq, 0xff, (size_t)d.p.nz * 4, b->stream));
+  if (invalidate_T(b, i)) return -1;
   HIPT(hipStreamSynchronize(b->stream));
   HIPT(hipFree(dx));
   return 0;
@@ -2553,6 +2704,7 @@ int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, cons
                      gbase, lab, rank);
   HIPT(hipGetLastError());
   HIPT(hipMemsetAsync(d.zliq, 0xff, (size_t)d.p.nz * 4, b->stream));
+  if (invalidate_T(b, i)) return -1;
   HIPT(hipMemcpyAsync(n_used, tot, sizeof(int), hipMemcpyDeviceToHost, b->stream));
   HIPT(hipStreamSynchronize(b->stream));
   HIPT(hipFree(dx));
