@@ -68,6 +68,9 @@ SIGNATURES = {
     "amcafe_sim_set_output": (_I, [_P, _I, _C]),
     "amcafe_write_hdf5_fields": (_I, [_C, _P, _P, _P, _P, _P, _P, _P, _I]),
     "amcafe_ipf_colours": (_I, [_P, _I, _P]),
+    "amcafe_heat_loss_f32": (_I, [_P, _P, _P, _I, _I, _P, _P]),
+    "amcafe_heat_loss_grad_f32": (_I, [_P, _P, _P, _I, _I, _P, _P, _P]),
+    "amcafe_heat_loss_f64": (_I, [_P, _P, _P, _I, _I, _P, _P]),
     "amcafe_sim_set_timing": (_I, [_P, _I]),
     "amcafe_sim_timings": (_I, [_P, _P, _I]),
 }

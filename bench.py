@@ -17,6 +17,9 @@ Also reported, in the same JSON line:
   batched       the 64 decks of decks/c2_batched (64^3 .. 256x128x96 voxels, one track,
                 1-2 layers) run to completion as one batch and then one by one: aggregate
                 cell-updates/s of each and the ratio
+  adjoint       the differentiable temperature on 256x256x64 voxels over 2000 steps: FP32
+                forward and forward + adjoint device times, their ratio, and the adjoint
+                gradient against FP64 central differences
 """
 import argparse
 import json
@@ -91,6 +94,32 @@ def batched_vs_single(deck_dir):
             "speedup": round(ts / tb, 3), "unit": "cell-updates/s"}
 
 
+def adjoint_cost(fd_steps=2000):
+    """The differentiation form: loss of the sigmoid melt indicator of the constant-property
+    moving source on 256x256x64 voxels (dx 2.5 um) over 2000 steps of one +x track, target
+    (P, v, eta, sigma) = (220 W, 0.9 m/s, 0.4, 45 um), evaluated at (200, 0.8, 0.35, 50 um):
+    FP32 forward time, FP32 forward + adjoint time, and the adjoint gradient against FP64
+    central differences (relative step 1e-4) over the first fd_steps steps."""
+    from amcafe_amd.heat import HeatField, track_desc
+    target, start = (220.0, 0.9, 0.4, 45e-6), (200.0, 0.8, 0.35, 50e-6)
+    f = HeatField(track_desc((256, 256, 64), steps=2000, checkpoint=100), target)
+    f.loss_f32(start, 0, 100)  # warm-up
+    t0 = time.perf_counter()
+    loss, ms_f = f.loss_f32(start, 0, 2000)
+    tf = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss_g, grad, ms_g = f.loss_grad_f32(start, 0, 2000)
+    tg = time.perf_counter() - t0
+    fd = f.central_differences(start, 0, fd_steps, h=1e-4)
+    if fd_steps != 2000:
+        _, grad, _ = f.loss_grad_f32(start, 0, fd_steps)
+    rel = [abs(g - d) / abs(d) for g, d in zip(grad, fd)]
+    return {"workload": "c3_heat", "voxels": 256 * 256 * 64, "steps": 2000, "dtype": "f32",
+            "loss": loss, "forward_ms": round(ms_f, 2), "forward_and_adjoint_ms": round(ms_g, 2),
+            "cost_ratio": round(ms_g / ms_f, 3), "wall_forward_s": round(tf, 3), "wall_adjoint_s": round(tg, 3),
+            "grad": grad, "fd_grad": fd, "fd_steps": fd_steps, "max_rel_err_vs_fd": max(rel)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +129,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-single comparison")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scale for HBM counters)")
     args = ap.parse_args()
@@ -185,6 +215,7 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_vs_single(os.path.join(ROOT, "decks", "c2_batched"))
+    adjoint = None if args.no_adjoint else adjoint_cost()
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
@@ -196,7 +227,7 @@ def main():
                    "events_per_step": round(tot["n_events"] / steps, 1),
                    "active_cells_per_step": round(tot["n_active"] / steps, 1),
                    "growing_cells_per_step": round(tot["n_growing"] / steps, 1)},
-        "roofline": roof, "cpu_baseline": cpu, "batched": batched,
+        "roofline": roof, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

@@ -159,6 +159,36 @@ int amcafe_ipf_colours(const double* cTheta, int nGrain, float* rgb9);
 int amcafe_sim_set_timing(amcafe_sim* s, int on);
 int amcafe_sim_timings(amcafe_sim* s, double* ms, int n);
 
+/* ---------------------------------------------------------------------------
+ * 3. Differentiable temperature field (no counterpart in the reference; for calibration
+ *    callers that fit beam parameters): the moving Gaussian source of Utilities::Temp_Fun_G
+ *    (S/Utilities.C:228-249) with constant properties on a straight +x track, integrated over
+ *    the last t_hist seconds with the 25-point Gauss-Legendre rule of Utilities::Integral, and
+ *    the loss sum over voxels and steps of (s(T) - s(T*))^2, s(T) = 1/(1+exp(-(T-tL)/width)),
+ *    T* the field of the target parameters.  theta = {P [W], v [m/s], eta, sigma [m]}.
+ *    Steps step0+1 .. step0+nsteps at t = n dt; each launch covers `checkpoint` steps (the field
+ *    of a step is recomputed from theta and t, nothing is stored between steps).  *ms: device
+ *    time (may be NULL).
+ * ------------------------------------------------------------------------- */
+typedef struct amcafe_heat_desc_s {
+  int nX[3];
+  double dX;
+  double x_start, y_track, z_track; /* source at (x_start + v t, y_track, z_track) */
+  double dt, t_hist;                /* step length; source-history window (8e-4 s in the reference) */
+  double T0, tL, width;             /* ambient, liquidus, indicator width */
+  double k, alpha;                  /* conductivity [W/m/K], diffusivity [m^2/s] */
+  int checkpoint;                   /* steps per launch (0: 100) */
+} amcafe_heat_desc;
+/* FP32 forward with FP32 accumulation */
+int amcafe_heat_loss_f32(const amcafe_heat_desc* d, const double* theta, const double* target, int step0,
+                         int nsteps, double* loss, float* ms);
+/* FP32 loss and its gradient with respect to theta (hand-written adjoint) */
+int amcafe_heat_loss_grad_f32(const amcafe_heat_desc* d, const double* theta, const double* target, int step0,
+                              int nsteps, double* loss, double* grad, float* ms);
+/* FP64 forward (finite differences) */
+int amcafe_heat_loss_f64(const amcafe_heat_desc* d, const double* theta, const double* target, int step0,
+                         int nsteps, double* loss, float* ms);
+
 #ifdef __cplusplus
 }
 #endif
