@@ -2142,7 +2142,12 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     delete b;
     return -1;
   }
-  for (int q = 0; q < 16; ++q) hipEventCreate(&b->ev[q]);
+  for (int q = 0; q < 16; ++q)
+    if (hipEventCreate(&b->ev[q]) != hipSuccess) {
+      snprintf(err, errlen, "hipEventCreate failed");
+      amcd_destroy(b);
+      return -1;
+    }
   b->bstartG.assign(n + 1, 0);
   b->bstartS.assign(n + 1, 0);
   b->bstartT.assign(n + 1, 0);
@@ -2176,17 +2181,27 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
       amcd_destroy(b);
       return -1;
     }
-    hipMemsetAsync(d.vs, 0, Nn, b->stream);
-    hipMemsetAsync(d.gid, 0, Nn * 4, b->stream);
-    hipMemsetAsync(d.ext, 0, Nn * 8, b->stream);
-    hipMemsetAsync(d.cx, 0, Nn * 8, b->stream);
-    hipMemsetAsync(d.cy, 0, Nn * 8, b->stream);
-    hipMemsetAsync(d.cz, 0, Nn * 8, b->stream);
-    hipMemsetAsync(d.T, 0, Nn * 8, b->stream);
-    hipMemsetAsync(d.bj, 0, Nn, b->stream);
-    hipMemsetAsync(d.zliq, 0xff, (size_t)p[i].nz * 4, b->stream);  // unknown: every plane may hold liquid
-    hipMemsetAsync(d.amap, 0xff, Nn * 4, b->stream);
-    hipMemsetAsync(d.ls, 0, sizeof(LoopState), b->stream);
+    hipError_t me = hipSuccess;
+    auto zero = [&](void* q, int val, size_t bytes) {
+      const hipError_t e = hipMemsetAsync(q, val, bytes, b->stream);
+      if (me == hipSuccess) me = e;
+    };
+    zero(d.vs, 0, Nn);
+    zero(d.gid, 0, Nn * 4);
+    zero(d.ext, 0, Nn * 8);
+    zero(d.cx, 0, Nn * 8);
+    zero(d.cy, 0, Nn * 8);
+    zero(d.cz, 0, Nn * 8);
+    zero(d.T, 0, Nn * 8);
+    zero(d.bj, 0, Nn);
+    zero(d.zliq, 0xff, (size_t)p[i].nz * 4);  // unknown: every plane may hold liquid
+    zero(d.amap, 0xff, Nn * 4);
+    zero(d.ls, 0, sizeof(LoopState));
+    if (me != hipSuccess) {
+      snprintf(err, errlen, "hipMemsetAsync: %s", hipGetErrorString(me));
+      amcd_destroy(b);
+      return -1;
+    }
     int acap = (int)std::min<long long>(N, std::max<long long>(1 << 16, N / 8));
     if (alloc_active(b, d, acap)) {
       snprintf(err, errlen, "%s", g_dev_err.c_str());
@@ -2218,11 +2233,16 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     amcd_destroy(b);
     return -1;
   }
-  hipMemcpyAsync(b->d_bstartG, b->bstartG.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
-  hipMemcpyAsync(b->d_bstartS, b->bstartS.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
-  hipMemcpyAsync(b->d_bstartT, b->bstartT.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
-  hipMemcpyAsync(b->d_cstart, b->cstart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
-  hipMemcpyAsync(b->d_bstartZ, b->bstartZ.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream);
+  const std::pair<int*, const std::vector<int>*> starts[] = {{b->d_bstartG, &b->bstartG}, {b->d_bstartS, &b->bstartS},
+                                                             {b->d_bstartT, &b->bstartT}, {b->d_cstart, &b->cstart},
+                                                             {b->d_bstartZ, &b->bstartZ}};
+  for (const auto& st : starts)
+    if (hipMemcpyAsync(st.first, st.second->data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream) !=
+        hipSuccess) {
+      snprintf(err, errlen, "hipMemcpyAsync of block starts failed");
+      amcd_destroy(b);
+      return -1;
+    }
   if (push_inst(b) || hipStreamSynchronize(b->stream) != hipSuccess) {
     snprintf(err, errlen, "%s", g_dev_err.c_str());
     amcd_destroy(b);
@@ -2232,15 +2252,15 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   return 0;
 }
 
-void amcd_destroy(amcd_batch* b) {
+void amcd_destroy(amcd_batch* b) {  // best effort: errors while tearing down are not reported
   if (!b) return;
-  if (b->stream) hipStreamSynchronize(b->stream);
-  for (void* q : b->allocs) hipFree(q);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (void* q : b->allocs) (void)hipFree(q);
   for (int q = 0; q < 16; ++q)
-    if (b->ev[q]) hipEventDestroy(b->ev[q]);
-  if (b->h_steps) hipHostFree(b->h_steps);
-  if (b->h_outs) hipHostFree(b->h_outs);
-  if (b->stream) hipStreamDestroy(b->stream);
+    if (b->ev[q]) (void)hipEventDestroy(b->ev[q]);
+  if (b->h_steps) (void)hipHostFree(b->h_steps);
+  if (b->h_outs) (void)hipHostFree(b->h_outs);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
@@ -2314,7 +2334,7 @@ static int ensure_grains(amcd_batch* b, Inst& d, int need) {
     HIPT(hipStreamSynchronize(b->stream));
     for (auto& q : b->allocs)
       if (q == d.rot) q = nr;
-    hipFree(d.rot);
+    HIPT(hipFree(d.rot));
   } else {
     b->allocs.push_back(nr);
   }
@@ -2365,7 +2385,7 @@ static int put_steps(amcd_batch* b, const amcd_step* steps) {
 }
 
 static void tmark(amcd_batch* b, int q) {
-  if (b->timing) hipEventRecord(b->ev[q], b->stream);
+  if (b->timing) (void)hipEventRecord(b->ev[q], b->stream);  // timing only
 }
 
 static int run_temperature(amcd_batch* b, int setliquid) {
@@ -2408,7 +2428,7 @@ int amcd_easm(amcd_batch* b, const amcd_step* steps, double Q, double layerT, am
   HIPT(hipFree(dn));
   if (b->timing) {
     float ms = 0;
-    hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
+    (void)hipEventElapsedTime(&ms, b->ev[0], b->ev[1]);
     b->last_ms[0] = ms;
   }
   if (n_evals) *n_evals = hn;
@@ -2518,10 +2538,10 @@ static int launch_finish(amcd_batch* b) {
 
 static void collect_timing(amcd_batch* b) {
   if (!b->timing) return;
-  hipEventSynchronize(b->ev[6]);
+  (void)hipEventSynchronize(b->ev[6]);
   for (int q = 0; q < 6; ++q) {
     float ms = 0;
-    hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]);
+    (void)hipEventElapsedTime(&ms, b->ev[q], b->ev[q + 1]);
     b->last_ms[q] = ms;
   }
 }

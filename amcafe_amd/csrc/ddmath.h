@@ -146,7 +146,7 @@ __host__ __device__ static inline void amc_atan2_fast(double y, double x, double
   const double qh = num / den;
   const double ql = fma(-qh, den, num) / den;  // num/den = qh + ql to 2^-106
   // 0/0 (a special case the callers replace) must still give an in-range table index
-  const int i = (int)rint(((qh >= 0.0) & (qh <= 1.0) ? qh : 0.0) * AMC_DD_TAB);
+  const int i = (int)rint((((qh >= 0.0) & (qh <= 1.0)) ? qh : 0.0) * AMC_DD_TAB);
   double rh, rl;
   if (i == 0) {
     rh = qh;
@@ -194,7 +194,7 @@ __host__ __device__ static inline double amc_atan2_q1(double y, double x) {
 __host__ __device__ static inline void amc_sincos_fast(double x, double* sh_out, double* sl_out, double* ch_out,
                                                        double* cl_out) {
   // arguments outside [0, pi/2] (from special cases the callers replace) still index the table
-  const int k = (int)rint(((x >= 0.0) & (x <= 1.58) ? x : 0.0) * AMC_DD_TAB);  // k <= 101
+  const int k = (int)rint((((x >= 0.0) & (x <= 1.58)) ? x : 0.0) * AMC_DD_TAB);  // k <= 101
   const double r = x - (double)k / AMC_DD_TAB;
   const double r2 = r * r;
   const double sm = r * r2 * (-0x1.5555555555555p-3 + r2 * (0x1.1111111111111p-7 + r2 * (-0x1.a01a01a01a01ap-13 +
