@@ -1208,7 +1208,10 @@ __device__ static inline double draw_batch(unsigned int x0, unsigned int cb, uns
 
 template <int NT, int K>
 __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
-                                  long long& visits, long long& evals) {
+                                  long long& visits_out, long long& evals_out) {
+  // work counters are kept in registers and handed back once at the end (a reference
+  // incremented inside the loop would live in scratch memory)
+  long long visits = 0, evals = 0;
   constexpr int NO = NT - 64;  // owner threads
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1390,6 +1393,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       if (s < ns && !(nc.valid && nc.slot == s)) d.gExt[s] = E[k];
     }
   }
+  visits_out += visits;
+  evals_out += evals;
 }
 
 // The owner protocol of owner_loop for growing lists larger than the register tiers hold:
@@ -1397,8 +1402,10 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
 // memory and every event streams the lane's cells (slots me, me + NO, ...).  Same
 // decisions, same arithmetic, same order of additions as the register tiers.
 template <int NT>
-__device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
-                                      long long& visits, long long& evals) {
+__device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst& d, EngineShared& sh,
+                                                                      const unsigned* dead, int go, double T,
+                                                                      long long& visits_out, long long& evals_out) {
+  long long visits = 0, evals = 0;
   constexpr int NO = NT - 64;
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1564,6 +1571,8 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
     skip = i1;
     par ^= 1;
   }
+  visits_out += visits;
+  evals_out += evals;
 }
 
 template <int NT, int K>
@@ -1631,6 +1640,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
       owner_loop_mem<NT>(d, sh, s_dead, go, ls->tinc, visits, evals);
   } else {
     // engine state, identical in every lane of wave 0
+    long long e_visits = 0, e_evals = 0;
     double tinc = ls->tinc;
     int vSind2 = ls->vSind2, nGrain = ls->nGrain, pool_used = ls->pool_used;
     int events = ls->events, captures = ls->captures, nucs = ls->nucs;
@@ -1790,8 +1800,8 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             }
           }
           nb = wave_best(nb);
-          visits++;
-          evals += __popcll(bal);
+          e_visits++;
+          e_evals += __popcll(bal);
           // the slot's static capture inputs (rotation, centroid, voxel)
           double rv = 0.0;
 #pragma unroll
@@ -1844,6 +1854,8 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
       ls->draws = draws0 + 2ll * nd;
       O[blockIdx.x].draws = draws0 + 2ll * nd;
     }
+    visits += e_visits;
+    evals += e_evals;
   }
   __syncthreads();
   // ---- loop state and counters
