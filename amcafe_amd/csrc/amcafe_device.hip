@@ -61,7 +61,7 @@ struct LoopState {
   int nGrain;
   int pool_used;
   unsigned int rng;  // minstd_rand0 state
-  int pad0;
+  int nslots_pre;    // slots whose lists the step's set-up built (-1 before the loop runs)
   long long draws;
   double tinc;
   int events, captures, nucs, done;
@@ -108,6 +108,7 @@ struct Inst {
   int2* gHd;        // per slot: list head (entry, target) of the memory-resident tier
   uint8_t *gN, *gDead, *gGrow;
   double *cD, *cF, *cR;
+  double4* cRC;     // per entry of a set-up list: the capture's re-centred centroid and extent
   int *cTgt, *cV;   // per entry: target (active index) and its voxel
   uint8_t* cK;
   // grains
@@ -640,6 +641,7 @@ __global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd
     ls->Na = Na;
     ls->nmushy0 = Nm;
     ls->nslots = 0;
+    ls->nslots_pre = -1;
     ls->vSind2 = Nm;
     ls->tinc = 0.0;
     ls->events = ls->captures = ls->nucs = 0;
@@ -823,54 +825,6 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
   d.gG[slot] = G;
 }
 
-// Candidate lists of the growing cells (S/VoxelsCA.C:396-440): one wave per slot, lane q
-// evaluates neighbour q when it is liquid, and the entries are placed by rank in
-// (D, ruqs, scan key) order -- the order the capture loop reads the list in.
-__device__ static inline void grow_list(const Inst& d, int slot, int lane) {
-  const int a = d.gAct[slot];
-  const int t = lane < AMCD_NB ? d.aNbr[(size_t)a * AMCD_NB + lane] : -1;
-  const bool liq = t >= 0 && d.aVS[t] == 1;
-  double D = __builtin_inf(), f = 0.0, r = __builtin_inf();
-  int key = 0x7fffffff, tv = -1;
-  if (liq) {
-    double c[3], R[9];
-    for (int q = 0; q < 3; ++q) c[q] = d.gC[3 * (size_t)slot + q];
-    for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)slot + q];
-    tv = d.aVox[t];
-    candidate(d, c, R, tv, &D, &f);
-    r = d.aRN[(size_t)a * AMCD_NB + lane];
-    key = d.gScan[slot] * 32 + lane;
-  }
-  const unsigned long long bal = __ballot(liq);
-  int rank = 0;
-  for (int j = 0; j < AMCD_NB; ++j) {
-    const double Dj = __shfl(D, j), rj = __shfl(r, j);
-    const int kj = __shfl(key, j);
-    if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
-  }
-  const size_t cb = (size_t)slot * AMCD_NB;
-  if (liq) {
-    d.cD[cb + rank] = D;
-    d.cF[cb + rank] = f;
-    d.cR[cb + rank] = r;
-    d.cTgt[cb + rank] = t;
-    d.cV[cb + rank] = tv;
-    d.cK[cb + rank] = (uint8_t)lane;
-  }
-  if (lane == 0) d.gN[slot] = (uint8_t)__popcll(bal);
-}
-// every wave of an instance's blocks takes slots wave, wave + (its waves), ...
-__global__ __launch_bounds__(GB) void k_grow_lists(Inst* I, const int* bstart, int ninst) {
-  int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
-  if (d.ls->done) return;
-  const int lane = threadIdx.x & 63;
-  const int nw = (bstart[ii + 1] - bstart[ii]) * (GB / 64);
-  const int ns = d.ls->nslots;
-  for (int slot = ((blockIdx.x - bstart[ii]) * GB + (int)threadIdx.x) >> 6; slot < ns; slot += nw)
-    if (!d.gDead[slot]) grow_list(d, slot, lane);
-}
-
 // Re-centring of the decentred octahedron after a capture, S/VoxelsCA.C:528-605 and
 // the helpers S/VoxelsCA.h:56-93.  R is row-major (local -> global), f is the
 // capturing candidate's shape factor.  Outputs the new centroid and half-diagonal.
@@ -959,6 +913,62 @@ __device__ static inline void recentre(const Inst& d, int vcap, int vnew, const 
   ncen[0] = cen[0] + locX[0];
   ncen[1] = cen[1] + locX[1];
   ncen[2] = cen[2] + locX[2];
+}
+
+// Candidate lists of the growing cells (S/VoxelsCA.C:396-440): one wave per slot, lane q
+// evaluates neighbour q when it is liquid, and the entries are placed by rank in
+// (D, ruqs, scan key) order -- the order the capture loop reads the list in.
+__device__ static inline void grow_list(const Inst& d, int slot, int lane) {
+  const int a = d.gAct[slot];
+  const int t = lane < AMCD_NB ? d.aNbr[(size_t)a * AMCD_NB + lane] : -1;
+  const bool liq = t >= 0 && d.aVS[t] == 1;
+  double D = __builtin_inf(), f = 0.0, r = __builtin_inf();
+  int key = 0x7fffffff, tv = -1;
+  double4 rc = make_double4(0.0, 0.0, 0.0, 0.0);
+  if (liq) {
+    double c[3], R[9];
+    for (int q = 0; q < 3; ++q) c[q] = d.gC[3 * (size_t)slot + q];
+    for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)slot + q];
+    tv = d.aVox[t];
+    candidate(d, c, R, tv, &D, &f);
+    r = d.aRN[(size_t)a * AMCD_NB + lane];
+    key = d.gScan[slot] * 32 + lane;
+    // the re-centring a capture through this entry performs depends on the entry and its
+    // cell alone (voxels, centroid, rotation, shape factor): evaluated here, in parallel,
+    // instead of on the capture loop's serial path
+    double nc[3], ne;
+    recentre(d, d.gV[slot], tv, c, R, f, nc, &ne);
+    rc = make_double4(nc[0], nc[1], nc[2], ne);
+  }
+  const unsigned long long bal = __ballot(liq);
+  int rank = 0;
+  for (int j = 0; j < AMCD_NB; ++j) {
+    const double Dj = __shfl(D, j), rj = __shfl(r, j);
+    const int kj = __shfl(key, j);
+    if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
+  }
+  const size_t cb = (size_t)slot * AMCD_NB;
+  if (liq) {
+    d.cD[cb + rank] = D;
+    d.cF[cb + rank] = f;
+    d.cR[cb + rank] = r;
+    d.cTgt[cb + rank] = t;
+    d.cV[cb + rank] = tv;
+    d.cK[cb + rank] = (uint8_t)lane;
+    d.cRC[cb + rank] = rc;
+  }
+  if (lane == 0) d.gN[slot] = (uint8_t)__popcll(bal);
+}
+// every wave of an instance's blocks takes slots wave, wave + (its waves), ...
+__global__ __launch_bounds__(GB) void k_grow_lists(Inst* I, const int* bstart, int ninst) {
+  int ii = find_inst(bstart, ninst, blockIdx.x);
+  const Inst& d = I[ii];
+  if (d.ls->done) return;
+  const int lane = threadIdx.x & 63;
+  const int nw = (bstart[ii + 1] - bstart[ii]) * (GB / 64);
+  const int ns = d.ls->nslots;
+  for (int slot = ((blockIdx.x - bstart[ii]) * GB + (int)threadIdx.x) >> 6; slot < ns; slot += nw)
+    if (!d.gDead[slot]) grow_list(d, slot, lane);
 }
 
 // The capture / nucleation loop of one instance (S/VoxelsCA.C:381-610): one workgroup
@@ -1612,6 +1622,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     }
   }
   if (tid == 0) {
+    if (ls->nslots_pre < 0) ls->nslots_pre = ls->nslots;  // the step's first launch
     sh.nslots = ls->nslots;
     sh.mb[0].valid = sh.mb[1].valid = 0;
     sh.susp[0] = sh.susp[1] = 0;
@@ -1645,6 +1656,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     int vSind2 = ls->vSind2, nGrain = ls->nGrain, pool_used = ls->pool_used;
     int events = ls->events, captures = ls->captures, nucs = ls->nucs;
     int nslots = ls->nslots;
+    const int ns_pre = ls->nslots_pre;
     // the uniforms, 64 per batch, drawn ahead by jump-ahead (S/VoxelsCA.C:483)
     const unsigned int x0 = ls->rng;
     const long long draws0 = ls->draws;
@@ -1709,13 +1721,19 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         const int v1 = cand >= 0 ? d.cV[cand] : d.aVox[i1];  // (no entry: a resumed nucleation)
         int vjs = 0, G = 1;
         double fcap = 0.0, R[9], c0[3], cen[3], E0;
+        // an entry of a list the set-up built carries its capture's re-centring
+        const bool pre = !x.nuc && sw < ns_pre;
+        double4 rc = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (pre) rc = d.cRC[cand];
         if (!x.nuc) {
 #pragma unroll
           for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)sw + q];
+          if (!pre) {
 #pragma unroll
-          for (int q = 0; q < 3; ++q) c0[q] = d.gC[3 * (size_t)sw + q];
-          fcap = d.cF[cand];
-          vjs = d.gV[sw];
+            for (int q = 0; q < 3; ++q) c0[q] = d.gC[3 * (size_t)sw + q];
+            fcap = d.cF[cand];
+            vjs = d.gV[sw];
+          }
           G = d.gG[sw];
         }
         if (x.nuc) {  // S/VoxelsCA.C:486-506
@@ -1730,7 +1748,14 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           nucs++;
           if (lane == 0) ls->pending = 0;
         } else {      // S/VoxelsCA.C:507-605
-          recentre(d, vjs, v1, c0, R, fcap, cen, &E0);
+          if (pre) {
+            cen[0] = rc.x;
+            cen[1] = rc.y;
+            cen[2] = rc.z;
+            E0 = rc.w;
+          } else {
+            recentre(d, vjs, v1, c0, R, fcap, cen, &E0);
+          }
           captures++;
         }
         const int scan = vSind2;
@@ -2116,7 +2141,8 @@ static int alloc_active(amcd_batch* b, Inst& d, int acap) {
       dalloc(b, &d.cD, c * AMCD_NB) || dalloc(b, &d.cF, c * AMCD_NB) || dalloc(b, &d.cR, c * AMCD_NB) ||
       dalloc(b, &d.cTgt, c * AMCD_NB) || dalloc(b, &d.cK, c * AMCD_NB) || dalloc(b, &d.aRN, c * AMCD_NB) ||
       dalloc(b, &d.gR, c * 9) || dalloc(b, &d.gC, c * 3) || dalloc(b, &d.gV, c) || dalloc(b, &d.gG, c) ||
-      dalloc(b, &d.cV, c * AMCD_NB) || dalloc(b, &d.gLH, c) || dalloc(b, &d.gHd, c))
+      dalloc(b, &d.cV, c * AMCD_NB) || dalloc(b, &d.gLH, c) || dalloc(b, &d.gHd, c) ||
+      dalloc(b, &d.cRC, c * AMCD_NB))
     return -1;
   return 0;
 }
@@ -2529,7 +2555,7 @@ static int launch_active(amcd_batch* b) {
   hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
                      b->d_outs);
   b->lstart[0] = 0;
-  for (int i = 0; i < n; ++i) b->lstart[i + 1] = b->lstart[i] + std::min(256, (b->h[i].acap + 3) / 4);
+  for (int i = 0; i < n; ++i) b->lstart[i + 1] = b->lstart[i] + std::min(2048, (b->h[i].acap + 3) / 4);
   HIPT(hipMemcpyAsync(b->d_lstart, b->lstart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
   hipLaunchKernelGGL(k_grow_lists, dim3(b->lstart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_lstart, n);
   tmark(b, 4);
