@@ -994,7 +994,6 @@ struct Best {
   double t, r;
   int key;   // scan position * 32 + neighbour rank: the reference's scan order
   int cand;  // candidate entry (slot * AMCD_NB + position)
-  int act;   // active index of the capturing cell
   int tgt;   // active index of the captured voxel
 };
 __device__ static inline Best best_none() {
@@ -1003,7 +1002,6 @@ __device__ static inline Best best_none() {
   b.r = __builtin_inf();
   b.key = 0x7fffffff;
   b.cand = -1;
-  b.act = -1;
   b.tgt = -1;
   return b;
 }
@@ -1021,7 +1019,6 @@ __device__ static inline Best shfl_best(const Best& b, int lane_mask) {
   o.r = __shfl_xor(b.r, lane_mask);
   o.key = __shfl_xor(b.key, lane_mask);
   o.cand = __shfl_xor(b.cand, lane_mask);
-  o.act = __shfl_xor(b.act, lane_mask);
   o.tgt = __shfl_xor(b.tgt, lane_mask);
   return o;
 }
@@ -1036,7 +1033,6 @@ __device__ static inline Best wave_best(Best b) {
     o.r = __shfl(b.r, src);
     o.key = __shfl(b.key, src);
     o.cand = __shfl(b.cand, src);
-    o.act = __shfl(b.act, src);
     o.tgt = __shfl(b.tgt, src);
     return o;
   }
@@ -1072,28 +1068,37 @@ __device__ static inline bool live_target(const unsigned* dead, int t, int skip)
 // first live one stay dead: a captured voxel never becomes liquid again within the
 // step.  `skip` is a voxel that counts as dead although its bit may not be set yet.
 struct SlotHead {
-  double D1;
-  int c1, tgt1;  // tgt1 = -1: no live entry
+  double D1, Dn, r1;  // head distance, the next entry's distance (+inf: none), head ruqs
+  int c1, tgt1, k1;   // head entry, its target (-1: no live entry), its neighbour rank
 };
+// First live entry at or after position `from` of slot s.  The entries are read four at
+// a time together with the list length, so the common case is one round of loads
+// (positions past the length hold stale data and are masked afterwards).
 __device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead, int s, int from, int skip) {
   SlotHead h;
   const int n = d.gN[s];
   const size_t cb = (size_t)s * AMCD_NB;
   h.c1 = (int)cb + n;
   h.tgt1 = -1;
-  h.D1 = __builtin_inf();
-  for (int p = from; p < n; p += 4) {
-    int t[4];
-    double D[4];
+  h.k1 = 0;
+  h.D1 = h.Dn = h.r1 = __builtin_inf();
+  for (int p = from; p < AMCD_NB; p += 4) {
+    int t[4], kq[4];
+    double D[5], R[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      t[q] = (p + q < n) ? d.cTgt[cb + p + q] : -1;
-      D[q] = (p + q < n) ? d.cD[cb + p + q] : 0.0;
+      const bool in = p + q < AMCD_NB;
+      t[q] = in ? d.cTgt[cb + p + q] : -1;
+      D[q] = in ? d.cD[cb + p + q] : 0.0;
+      R[q] = in ? d.cR[cb + p + q] : 0.0;
+      kq[q] = in ? d.cK[cb + p + q] : 0;
     }
+    D[4] = (p + 4 < AMCD_NB) ? d.cD[cb + p + 4] : 0.0;
+    if (p >= n) break;
     int hit = -1;
 #pragma unroll
     for (int q = 3; q >= 0; --q)
-      if (live_target(dead, t[q], skip)) hit = q;
+      if (p + q < n && live_target(dead, t[q], skip)) hit = q;
     if (hit >= 0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -1101,6 +1106,9 @@ __device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead,
           h.c1 = (int)cb + p + q;
           h.tgt1 = t[q];
           h.D1 = D[q];
+          h.Dn = (p + q + 1 < n) ? D[q + 1] : __builtin_inf();
+          h.r1 = R[q];
+          h.k1 = kq[q];
         }
       break;
     }
@@ -1138,8 +1146,8 @@ __device__ static inline void tau_bounds(double D1, double e, double v, double T
 }
 
 struct NewCell {  // a growing cell created by the engine, taken over by its owner
-  double E, V, D1;
-  int c1, tgt1, slot, valid;
+  double E, V, D1, Dn1, r1;  // extent, velocity, head distance, next distance, head ruqs
+  int c1, tgt1, key1, slot, valid;
 };
 struct EngineShared {
   Best sb[2][16];   // per-wave bests, by event parity
@@ -1152,7 +1160,7 @@ struct EngineShared {
 };
 // The decision every wave takes after the barrier (S/VoxelsCA.C:476-485).
 struct Decision {
-  int stop, nuc, i1, cand, js, wave;
+  int stop, nuc, i1, cand, wave;
   double dt;
 };
 // The table is read in full before any comparison, so the reads overlap; `nd` is the
@@ -1181,7 +1189,6 @@ __device__ static inline Decision decide(const EngineShared& sh, int par, double
       e[q].r = bt ? o.r : e[q].r;
       e[q].key = bt ? o.key : e[q].key;
       e[q].cand = bt ? o.cand : e[q].cand;
-      e[q].act = bt ? o.act : e[q].act;
       e[q].tgt = bt ? o.tgt : e[q].tgt;
       wi[q] = bt ? wi[q + h] : wi[q];
     }
@@ -1193,7 +1200,6 @@ __device__ static inline Decision decide(const EngineShared& sh, int par, double
   x.nuc = 0;
   x.i1 = b.tgt;
   x.cand = b.cand;
-  x.js = b.act;
   x.wave = wq;
   x.dt = 0.0;
   if (!(T < DelT)) {
@@ -1216,9 +1222,18 @@ __device__ static inline double draw_batch(unsigned int x0, unsigned int cb, uns
   return canonical(x1, lcg_next(x1));
 }
 
+// Tiers of up to eight cells per owner thread keep each cell's head (distance, next
+// distance, ruqs) in LDS and its scan key in a register, so that EVAL reads no global
+// memory unless a tie has to be walked; larger tiers read the head from the lists.
+template <int K>
+struct HeadCache {
+  static constexpr bool on = K > 0 && K <= 8;
+};
 template <int NT, int K>
-__device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigned* dead, int go, double T,
-                                  long long& visits_out, long long& evals_out) {
+__device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigned* dead, double* hc, int go,
+                                  double T, long long& visits_out, long long& evals_out) {
+  constexpr bool HC = HeadCache<K>::on;
+  constexpr int HN = (NT - 64) * K;  // head-cache slots (one per cell of the tier)
   // work counters are kept in registers and handed back once at the end (a reference
   // incremented inside the loop would live in scratch memory)
   long long visits = 0, evals = 0;
@@ -1230,6 +1245,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   const amcd_params& p = d.p;
   double E[K], V[K];
   float lo[K], hi[K];
+  // c1: the head entry; with the head cache, its position in the slot's list (5 bits)
+  // packed with its scan key (scan position * 32 + neighbour rank, < 2^24) above them
   int c1[K], tgt1[K];
   const bool loaded = go != -2;
   const int ns0 = sh.nslots;
@@ -1246,6 +1263,12 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       const SlotHead h = slot_head(d, dead, s, 0, -1);
       c1[k] = h.c1;
       tgt1[k] = h.tgt1;
+      if constexpr (HC) {
+        c1[k] = (h.c1 - s * AMCD_NB) | ((d.gScan[s] * 32 + h.k1) << 5);
+        hc[s] = h.D1;
+        hc[HN + s] = h.Dn;
+        hc[2 * HN + s] = h.r1;
+      }
       if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
     }
   }
@@ -1278,15 +1301,24 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         if (lo[k] != fInf && lo[k] <= u) {
           // exact capture time of the cell's head (S/VoxelsCA.C:420-440)
           const int s = mo + k * NO;
-          const int c = c1[k];
+          const int c = HC ? s * AMCD_NB + (c1[k] & 31) : c1[k];
           const int pos = c - s * AMCD_NB;
           const double e = E[k], v = V[k];
-          const double D1 = d.cD[c];
-          const double Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
-          const double rr = d.cR[c];
-          const int kb = d.gScan[s] * 32;
-          const int kk = d.cK[c];
-          const int a = d.gAct[s];
+          double D1, Dn, rr;
+          int kb, ky;
+          if constexpr (HC) {
+            D1 = hc[s];
+            Dn = hc[HN + s];
+            rr = hc[2 * HN + s];
+            ky = c1[k] >> 5;
+            kb = ky & ~31;
+          } else {
+            D1 = d.cD[c];
+            Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
+            rr = d.cR[c];
+            kb = d.gScan[s] * 32;
+            ky = kb + d.cK[c];
+          }
           visits++;
           evals++;
           const double t1 = (D1 - e) / v;
@@ -1294,9 +1326,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
             Best y;
             y.t = t1;
             y.r = rr;
-            y.key = kb + kk;
+            y.key = ky;
             y.cand = c;
-            y.act = a;
             y.tgt = tgt1[k];
             // times are non-decreasing along the list: a tie needs the next entry's
             // time to equal t1; then walk the tied block
@@ -1313,7 +1344,6 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
                 z.r = d.cR[c - pos + q];
                 z.key = kb + d.cK[c - pos + q];
                 z.cand = c - pos + q;
-                z.act = a;
                 z.tgt = tq;
                 if (better(z, y)) y = z;
               }
@@ -1365,10 +1395,15 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         if (k == kk) {
           E[k] = e;
           V[k] = nc.V;
-          c1[k] = nc.c1;
+          c1[k] = HC ? nc.key1 << 5 : nc.c1;  // the head is entry 0
           tgt1[k] = nc.tgt1;
           lo[k] = -fInf;  // bounds below (its head may also be the voxel captured now)
         }
+      if constexpr (HC) {
+        hc[nc.slot] = nc.D1;
+        hc[HN + nc.slot] = nc.Dn1;
+        hc[2 * HN + nc.slot] = nc.r1;
+      }
     }
     // heads captured now, clamped cells, the new cell, and every cell once per 2^20 events
     ++nev;
@@ -1380,13 +1415,21 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       if (!hit) continue;
       const int s = mo + k * NO;
       if (tgt1[k] == i1) {
-        const SlotHead h = slot_head(d, dead, s, c1[k] - s * AMCD_NB + 1, i1);
-        c1[k] = h.c1;
+        const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
+        const SlotHead h = slot_head(d, dead, s, pos + 1, i1);
         tgt1[k] = h.tgt1;
+        if constexpr (HC) {
+          c1[k] = (h.c1 - s * AMCD_NB) | ((((c1[k] >> 5) & ~31) + h.k1) << 5);
+          hc[s] = h.D1;
+          hc[HN + s] = h.Dn;
+          hc[2 * HN + s] = h.r1;
+        } else {
+          c1[k] = h.c1;
+        }
         lo[k] = hi[k] = fInf;
         if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
       } else {
-        tau_bounds(d.cD[c1[k]], E[k], V[k], T, lo[k], hi[k]);
+        tau_bounds(HC ? hc[s] : d.cD[c1[k]], E[k], V[k], T, lo[k], hi[k]);
       }
     }
     skip = i1;
@@ -1475,7 +1518,6 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
           const double rr = d.cR[c];
           const int kb = d.gScan[s] * 32;
           const int kk = d.cK[c];
-          const int a = d.gAct[s];
           visits++;
           evals++;
           const double t1 = (D1 - e) / v;
@@ -1485,7 +1527,6 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
             y.r = rr;
             y.key = kb + kk;
             y.cand = c;
-            y.act = a;
             y.tgt = hd.y;
             if ((Dn - e) / v == t1) {
               const int n = d.gN[s];
@@ -1500,7 +1541,6 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
                 z.r = d.cR[c - pos + q];
                 z.key = kb + d.cK[c - pos + q];
                 z.cand = c - pos + q;
-                z.act = a;
                 z.tgt = tq;
                 if (better(z, y)) y = z;
               }
@@ -1594,6 +1634,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   __shared__ EngineShared sh;
   __shared__ int s_go;
   __shared__ unsigned s_dead[AMCD_DEAD_BITS / 32];
+  __shared__ double s_hc[HeadCache<K>::on ? 3 * (NT - 64) * K : 1];
   // The descriptor and step record are copied to LDS: every event ends in a barrier, after
   // which the compiler must re-read anything in global memory, and each array access would
   // otherwise wait for its base pointer to come back from global memory first.
@@ -1646,7 +1687,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   long long visits = 0, evals = 0;
   if (w > 0) {
     if constexpr (K > 0)
-      owner_loop<NT, K>(d, sh, s_dead, go, ls->tinc, visits, evals);
+      owner_loop<NT, K>(d, sh, s_dead, s_hc, go, ls->tinc, visits, evals);
     else
       owner_loop_mem<NT>(d, sh, s_dead, go, ls->tinc, visits, evals);
   } else {
@@ -1673,7 +1714,6 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     x.nuc = 1;
     x.i1 = sh.r_i1;
     x.cand = -1;
-    x.js = -1;
     x.wave = 0;
     x.dt = 0.0;
     bool apply_first = go == 2;
@@ -1810,9 +1850,13 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             d.cK[cb + rank] = (uint8_t)lane;
             if (rank == 0) {
               nc.D1 = D;
+              nc.r1 = r;
+              nc.key1 = key;
               nc.c1 = (int)cb;
               nc.tgt1 = tn;
+              if (__popcll(bal) < 2) nc.Dn1 = __builtin_inf();
             }
+            if (rank == 1) nc.Dn1 = D;
             // this cell's exact best for the next event (all its targets are liquid)
             const double tq = (D - E0) / V0;
             if (tq <= 1e6) {
@@ -1820,7 +1864,6 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
               nb.r = r;
               nb.key = key;
               nb.cand = (int)(cb + rank);
-              nb.act = i1;
               nb.tgt = tn;
             }
           }
