@@ -1163,38 +1163,67 @@ struct Decision {
   int stop, nuc, i1, cand, wave;
   double dt;
 };
-// The table is read in full before any comparison, so the reads overlap; `nd` is the
-// number of draws this launch has consumed, which selects the event's uniform.
+// Lane q < NW reads wave q's best (one entry per lane, so the eight waves that decide at
+// once after the barrier do not queue on the LDS for the whole table each), and the
+// tournament runs across lanes with row shifts: at level h lane q compares its entry
+// with lane q + h's.  `nd` is the number of draws this launch has consumed, which
+// selects the event's uniform.
+template <int H>
+__device__ static inline int row_from(int v) {  // lane i takes lane i + H of its row of 16
+  return __builtin_amdgcn_update_dpp(v, v, 0x100 + H, 0xf, 0xf, false);
+}
+template <int H>
+__device__ static inline double row_from(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)row_from<H>((int)b), hi = (unsigned)row_from<H>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ static inline double lane0(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 0);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 0);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <int NW, int H>
+__device__ static inline void tournament_level(Best& e, int& wi) {
+  if constexpr (H < NW) {
+    Best o;
+    o.t = row_from<H>(e.t);
+    o.r = row_from<H>(e.r);
+    o.key = row_from<H>(e.key);
+    o.cand = row_from<H>(e.cand);
+    o.tgt = row_from<H>(e.tgt);
+    const int ow = row_from<H>(wi);
+    const bool bt = better(o, e);
+    e.t = bt ? o.t : e.t;
+    e.r = bt ? o.r : e.r;
+    e.key = bt ? o.key : e.key;
+    e.cand = bt ? o.cand : e.cand;
+    e.tgt = bt ? o.tgt : e.tgt;
+    wi = bt ? ow : wi;
+    tournament_level<NW, 2 * H>(e, wi);
+  }
+}
 template <int NW>
 __device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX, int nd) {
-  Best e[NW];
-  int wi[NW];
-#pragma unroll
-  for (int q = 0; q < NW; ++q) {
-    e[q] = sh.sb[par][q];
-    wi[q] = q;
-  }
+  static_assert(NW <= 16, "one row of lanes");
+  const int lane = threadIdx.x & 63;
+  Best e = best_none();
+  if (lane < NW) e = sh.sb[par][lane];
+  int wi = lane;
   const double U = sh.ubuf[(nd >> 6) & 1][nd & 63];
   const int susp = sh.susp[par];
   // pairwise tree, the left entry kept unless the right one is strictly better: the
   // first best entry in wave order, as a sequential scan finds it (the order is a
   // strict weak order: times and ruqs are never NaN here)
-#pragma unroll
-  for (int h = 1; h < NW; h <<= 1) {
-#pragma unroll
-    for (int q = 0; q + h < NW; q += 2 * h) {
-      const Best& o = e[q + h];
-      const bool bt = better(o, e[q]);
-      e[q].t = bt ? o.t : e[q].t;
-      e[q].r = bt ? o.r : e[q].r;
-      e[q].key = bt ? o.key : e[q].key;
-      e[q].cand = bt ? o.cand : e[q].cand;
-      e[q].tgt = bt ? o.tgt : e[q].tgt;
-      wi[q] = bt ? wi[q + h] : wi[q];
-    }
-  }
-  const Best b = e[0];
-  const int wq = wi[0];
+  tournament_level<NW, 1>(e, wi);
+  Best b;
+  b.t = lane0(e.t);
+  b.r = lane0(e.r);
+  b.key = __builtin_amdgcn_readlane(e.key, 0);
+  b.cand = __builtin_amdgcn_readlane(e.cand, 0);
+  b.tgt = __builtin_amdgcn_readlane(e.tgt, 0);
+  const int wq = __builtin_amdgcn_readlane(wi, 0);
   Decision x;
   x.stop = 0;
   x.nuc = 0;
