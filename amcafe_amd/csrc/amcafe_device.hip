@@ -1013,34 +1013,78 @@ __device__ static inline bool better(const Best& a, const Best& b) {
   const bool tl = a.t < b.t, tg = a.t > b.t, rl = a.r < b.r, rg = a.r > b.r;
   return tl | (!tg & (rl | (!rg & (a.key < b.key))));
 }
-__device__ static inline Best shfl_best(const Best& b, int lane_mask) {
+__device__ static inline double lane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// Wave-wide reductions by DPP row operations: two quad permutes, the half-row and row
+// mirrors leave a row's result in all 16 of its lanes, row_bcast:15 and row_bcast:31
+// carry rows 0 and 2 into rows 1 and 3 and then row 1 into rows 2 and 3, so that lane
+// 63 ends with the result for the whole wave.  Every lane must be active.
+template <int CTRL, int ROWS = 0xf>
+__device__ static inline int dpp(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ static inline float dpp(float v) {
+  return __int_as_float(dpp<CTRL, ROWS>(__float_as_int(v)));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ static inline double dpp(double v) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)dpp<CTRL, ROWS>((int)b), hi = (unsigned)dpp<CTRL, ROWS>((int)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+constexpr int kQuadX1 = 0xb1, kQuadX2 = 0x4e, kHalfMirror = 0x141, kMirror = 0x140, kBcast15 = 0x142,
+              kBcast31 = 0x143;
+__device__ static inline float wave_min(float v) {
+  v = fminf(v, dpp<kQuadX1>(v));
+  v = fminf(v, dpp<kQuadX2>(v));
+  v = fminf(v, dpp<kHalfMirror>(v));
+  v = fminf(v, dpp<kMirror>(v));
+  v = fminf(v, dpp<kBcast15, 0xa>(v));
+  v = fminf(v, dpp<kBcast31, 0xc>(v));
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ static inline void best_step(Best& b) {
   Best o;
-  o.t = __shfl_xor(b.t, lane_mask);
-  o.r = __shfl_xor(b.r, lane_mask);
-  o.key = __shfl_xor(b.key, lane_mask);
-  o.cand = __shfl_xor(b.cand, lane_mask);
-  o.tgt = __shfl_xor(b.tgt, lane_mask);
+  o.t = dpp<CTRL, ROWS>(b.t);
+  o.r = dpp<CTRL, ROWS>(b.r);
+  o.key = dpp<CTRL, ROWS>(b.key);
+  o.cand = dpp<CTRL, ROWS>(b.cand);
+  o.tgt = dpp<CTRL, ROWS>(b.tgt);
+  const bool bt = better(o, b);
+  b.t = bt ? o.t : b.t;
+  b.r = bt ? o.r : b.r;
+  b.key = bt ? o.key : b.key;
+  b.cand = bt ? o.cand : b.cand;
+  b.tgt = bt ? o.tgt : b.tgt;
+}
+__device__ static inline Best lane_best(const Best& b, int l) {
+  Best o;
+  o.t = lane_f64(b.t, l);
+  o.r = lane_f64(b.r, l);
+  o.key = __builtin_amdgcn_readlane(b.key, l);
+  o.cand = __builtin_amdgcn_readlane(b.cand, l);
+  o.tgt = __builtin_amdgcn_readlane(b.tgt, l);
   return o;
 }
-// best over a wave; a wave with at most one candidate needs no shuffles
+// best over a wave.  (t, ruqs, key) is a strict total order on candidates -- keys are
+// unique -- so the order in which lanes are combined does not change the result.
 __device__ static inline Best wave_best(Best b) {
   const unsigned long long m = __ballot(b.cand >= 0);
   if (m == 0) return best_none();
-  if (__popcll(m) == 1) {
-    const int src = __builtin_ctzll(m);
-    Best o;
-    o.t = __shfl(b.t, src);
-    o.r = __shfl(b.r, src);
-    o.key = __shfl(b.key, src);
-    o.cand = __shfl(b.cand, src);
-    o.tgt = __shfl(b.tgt, src);
-    return o;
-  }
-  for (int s = 32; s >= 1; s >>= 1) {
-    Best o = shfl_best(b, s);
-    if (better(o, b)) b = o;
-  }
-  return b;
+  if (__popcll(m) == 1) return lane_best(b, __builtin_ctzll(m));
+  best_step<kQuadX1>(b);
+  best_step<kQuadX2>(b);
+  best_step<kHalfMirror>(b);
+  best_step<kMirror>(b);
+  best_step<kBcast15, 0xa>(b);
+  best_step<kBcast31, 0xc>(b);
+  return lane_best(b, 63);
 }
 // candidate order inside a growing cell's list: (D, ruqs, key), NaN distances last
 // (branch-free for the reason given at better())
@@ -1178,12 +1222,6 @@ __device__ static inline double row_from(double v) {
   const unsigned lo = (unsigned)row_from<H>((int)b), hi = (unsigned)row_from<H>((int)(b >> 32));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
-__device__ static inline double lane0(double v) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 0);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 0);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
 template <int NW, int H>
 __device__ static inline void tournament_level(Best& e, int& wi) {
   if constexpr (H < NW) {
@@ -1218,8 +1256,8 @@ __device__ static inline Decision decide(const EngineShared& sh, int par, double
   // strict weak order: times and ruqs are never NaN here)
   tournament_level<NW, 1>(e, wi);
   Best b;
-  b.t = lane0(e.t);
-  b.r = lane0(e.r);
+  b.t = lane_f64(e.t, 0);
+  b.r = lane_f64(e.r, 0);
   b.key = __builtin_amdgcn_readlane(e.key, 0);
   b.cand = __builtin_amdgcn_readlane(e.cand, 0);
   b.tgt = __builtin_amdgcn_readlane(e.tgt, 0);
@@ -1323,7 +1361,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       float u = fInf;
 #pragma unroll
       for (int k = 0; k < K; ++k) u = fminf(u, hi[k]);
-      for (int m = 32; m >= 1; m >>= 1) u = fminf(u, __shfl_xor(u, m));
+      u = wave_min(u);
       Best b = best_none();
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -1532,7 +1570,7 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
           lmin = fminf(lmin, lh[q].x);
         }
       }
-      for (int m = 32; m >= 1; m >>= 1) u = fminf(u, __shfl_xor(u, m));
+      u = wave_min(u);
       Best b = best_none();
       if (lmin <= u) {
         for (int s = me; s < ns; s += NO) {
