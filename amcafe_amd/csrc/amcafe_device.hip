@@ -49,6 +49,15 @@ __device__ unsigned long long g_phase[16];
 #define PHASE(v, slot, pred)
 #endif
 
+// A pointer known to address global memory.  The capture loop reads its array pointers
+// from a descriptor copied to LDS, so the compiler cannot tell their address space and
+// would emit flat accesses, which also count against the LDS wait counter: every LDS
+// wait would then wait for the outstanding memory loads as well.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 #define ZB 1024     // voxels per compaction block (one per thread)
 #define GB 256      // threads per grid-wide block
 #define SB 4096     // voxels per solid->mushy block (GB threads, 16 voxels each)
@@ -1120,7 +1129,7 @@ struct SlotHead {
 // (positions past the length hold stale data and are masked afterwards).
 __device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead, int s, int from, int skip) {
   SlotHead h;
-  const int n = d.gN[s];
+  const int n = gmem(d.gN)[s];
   const size_t cb = (size_t)s * AMCD_NB;
   h.c1 = (int)cb + n;
   h.tgt1 = -1;
@@ -1132,12 +1141,12 @@ __device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const bool in = p + q < AMCD_NB;
-      t[q] = in ? d.cTgt[cb + p + q] : -1;
-      D[q] = in ? d.cD[cb + p + q] : 0.0;
-      R[q] = in ? d.cR[cb + p + q] : 0.0;
-      kq[q] = in ? d.cK[cb + p + q] : 0;
+      t[q] = in ? gmem(d.cTgt)[cb + p + q] : -1;
+      D[q] = in ? gmem(d.cD)[cb + p + q] : 0.0;
+      R[q] = in ? gmem(d.cR)[cb + p + q] : 0.0;
+      kq[q] = in ? gmem(d.cK)[cb + p + q] : 0;
     }
-    D[4] = (p + 4 < AMCD_NB) ? d.cD[cb + p + 4] : 0.0;
+    D[4] = (p + 4 < AMCD_NB) ? gmem(d.cD)[cb + p + 4] : 0.0;
     if (p >= n) break;
     int hit = -1;
 #pragma unroll
@@ -1243,11 +1252,17 @@ __device__ static inline void tournament_level(Best& e, int& wi) {
   }
 }
 template <int NW>
-__device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX, int nd) {
+__device__ static inline Best lane_entry(const EngineShared& sh, int par) {  // wave `lane`'s best
   static_assert(NW <= 16, "one row of lanes");
   const int lane = threadIdx.x & 63;
   Best e = best_none();
   if (lane < NW) e = sh.sb[par][lane];
+  return e;
+}
+template <int NW>
+__device__ static inline Decision decide_from(const EngineShared& sh, Best e, int par, double T, double DelT,
+                                              double rX, int nd) {
+  const int lane = threadIdx.x & 63;
   int wi = lane;
   const double U = sh.ubuf[(nd >> 6) & 1][nd & 63];
   const int susp = sh.susp[par];
@@ -1280,6 +1295,75 @@ __device__ static inline Decision decide(const EngineShared& sh, int par, double
     x.dt = x.nuc ? 0.0 : b.t;
   }
   return x;
+}
+template <int NW>
+__device__ static inline Decision decide(const EngineShared& sh, int par, double T, double DelT, double rX, int nd) {
+  return decide_from<NW>(sh, lane_entry<NW>(sh, par), par, T, DelT, rX, nd);
+}
+// What the engine needs to perform a capture, loaded for every wave's best while the
+// decision is being taken: lane q < NW holds the entry of wave q's best (target voxel,
+// capturing cell's grain and rotation, the target's cold flag and temperature, the
+// set-up's re-centring), and lane l holds positions l, l + 64, l + 128, l + 192 of the
+// candidates' neighbour rows laid end to end (26 per candidate).
+struct EnginePrefetch {
+  double R[9], T1, rpn[4];
+  double4 rc;
+  int v1, G, cold, nbr[4];
+};
+template <int NW>
+__device__ static inline EnginePrefetch engine_prefetch(const Inst& d, const EngineShared& sh, const Best& e, int par,
+                                                        int ns_pre) {
+  // branch-free, with every index clamped into its array (a lane without an entry reads
+  // entry 0), so that the loads are issued here and not moved past the decision
+  const int lane = threadIdx.x & 63;
+  EnginePrefetch f;
+  const int cand = e.cand >= 0 ? e.cand : 0, tgt = e.tgt >= 0 ? e.tgt : 0;
+  const size_t sw = (size_t)(cand / AMCD_NB);
+  f.v1 = gmem(d.cV)[cand];
+  f.G = gmem(d.gG)[sw];
+  f.cold = gmem(d.aCold)[tgt];
+  f.T1 = gmem(d.aT)[tgt];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) f.R[q] = gmem(d.gR)[9 * sw + q];
+  {  // meaningful for entries of set-up lists only
+    const auto q = gmem((double*)(d.cRC + cand));
+    f.rc = make_double4(q[0], q[1], q[2], q[3]);
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const int fl = lane + 64 * m, q = min(fl / AMCD_NB, NW - 1), j = fl - (fl / AMCD_NB) * AMCD_NB;
+    int tq = sh.sb[par][q].tgt;
+    tq = tq >= 0 ? tq : 0;
+    f.nbr[m] = gmem(d.aNbr)[(size_t)tq * AMCD_NB + j];
+    f.rpn[m] = gmem(d.aRN)[(size_t)tq * AMCD_NB + j];
+  }
+  return f;
+}
+// keeps the prefetched registers live at this point: the loads must have been issued
+// before it, and their wait is placed here
+__device__ static inline void prefetch_landed(const EnginePrefetch& f) {
+  asm volatile("" ::"v"(f.v1), "v"(f.G), "v"(f.cold), "v"(f.T1), "v"(f.rc.x), "v"(f.rc.y), "v"(f.rc.z),
+               "v"(f.rc.w), "v"(f.nbr[0]), "v"(f.nbr[1]), "v"(f.nbr[2]), "v"(f.nbr[3]), "v"(f.rpn[0]),
+               "v"(f.rpn[1]), "v"(f.rpn[2]), "v"(f.rpn[3]));
+  asm volatile("" ::"v"(f.R[0]), "v"(f.R[1]), "v"(f.R[2]), "v"(f.R[3]), "v"(f.R[4]), "v"(f.R[5]), "v"(f.R[6]),
+               "v"(f.R[7]), "v"(f.R[8]));
+}
+// lane j < AMCD_NB takes position j of wave wq's neighbour row from the prefetch
+__device__ static inline void take_row(const EnginePrefetch& f, int wq, int lane, int& tn, double& rp) {
+  const int fl = wq * AMCD_NB + (lane < AMCD_NB ? lane : 0), src = (fl & 63) << 2, m = fl >> 6;
+  int t = -1, lo = 0, hi = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int a = __builtin_amdgcn_ds_bpermute(src, f.nbr[k]);
+    const long long b = __double_as_longlong(f.rpn[k]);
+    const int bl = __builtin_amdgcn_ds_bpermute(src, (int)b), bh = __builtin_amdgcn_ds_bpermute(src, (int)(b >> 32));
+    t = m == k ? a : t;
+    lo = m == k ? bl : lo;
+    hi = m == k ? bh : hi;
+  }
+  tn = lane < AMCD_NB ? t : -1;
+  rp = lane < AMCD_NB ? __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo))
+                      : 0.0;
 }
 // Uniforms 64b .. 64b+63 of a launch whose engine state was x0 at its start: lane j
 // jumps to x0 * 16807^(128b + 2j + 1) (aj = 16807^(2j+1), cb = 16807^(128b)).
@@ -1325,13 +1409,13 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
     c1[k] = 0;
     tgt1[k] = -1;
     if (loaded && s < ns0) {
-      E[k] = d.gExt[s];
-      V[k] = d.gVhat[s];
+      E[k] = gmem(d.gExt)[s];
+      V[k] = gmem(d.gVhat)[s];
       const SlotHead h = slot_head(d, dead, s, 0, -1);
       c1[k] = h.c1;
       tgt1[k] = h.tgt1;
       if constexpr (HC) {
-        c1[k] = (h.c1 - s * AMCD_NB) | ((d.gScan[s] * 32 + h.k1) << 5);
+        c1[k] = (h.c1 - s * AMCD_NB) | ((gmem(d.gScan)[s] * 32 + h.k1) << 5);
         hc[s] = h.D1;
         hc[HN + s] = h.Dn;
         hc[2 * HN + s] = h.r1;
@@ -1380,11 +1464,11 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
             ky = c1[k] >> 5;
             kb = ky & ~31;
           } else {
-            D1 = d.cD[c];
-            Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
-            rr = d.cR[c];
-            kb = d.gScan[s] * 32;
-            ky = kb + d.cK[c];
+            D1 = gmem(d.cD)[c];
+            Dn = (pos + 1 < AMCD_NB) ? gmem(d.cD)[c + 1] : __builtin_inf();
+            rr = gmem(d.cR)[c];
+            kb = gmem(d.gScan)[s] * 32;
+            ky = kb + gmem(d.cK)[c];
           }
           visits++;
           evals++;
@@ -1399,17 +1483,17 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
             // times are non-decreasing along the list: a tie needs the next entry's
             // time to equal t1; then walk the tied block
             if ((Dn - e) / v == t1) {
-              const int n = d.gN[s];
+              const int n = gmem(d.gN)[s];
               for (int q = pos + 1; q < n; ++q) {
-                const int tq = d.cTgt[c - pos + q];
+                const int tq = gmem(d.cTgt)[c - pos + q];
                 if (!live_target(dead, tq, skip)) continue;
                 evals++;
-                const double tt = (d.cD[c - pos + q] - e) / v;
+                const double tt = (gmem(d.cD)[c - pos + q] - e) / v;
                 if (!(tt == t1)) break;
                 Best z;
                 z.t = tt;
-                z.r = d.cR[c - pos + q];
-                z.key = kb + d.cK[c - pos + q];
+                z.r = gmem(d.cR)[c - pos + q];
+                z.key = kb + gmem(d.cK)[c - pos + q];
                 z.cand = c - pos + q;
                 z.tgt = tq;
                 if (better(z, y)) y = z;
@@ -1496,7 +1580,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         lo[k] = hi[k] = fInf;
         if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
       } else {
-        tau_bounds(HC ? hc[s] : d.cD[c1[k]], E[k], V[k], T, lo[k], hi[k]);
+        tau_bounds(HC ? hc[s] : gmem(d.cD)[c1[k]], E[k], V[k], T, lo[k], hi[k]);
       }
     }
     skip = i1;
@@ -1510,7 +1594,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int s = me + k * NO;
-      if (s < ns && !(nc.valid && nc.slot == s)) d.gExt[s] = E[k];
+      if (s < ns && !(nc.valid && nc.slot == s)) gmem(d.gExt)[s] = E[k];
     }
   }
   visits_out += visits;
@@ -1784,13 +1868,21 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     x.wave = 0;
     x.dt = 0.0;
     bool apply_first = go == 2;
+    EnginePrefetch pf;
+    bool have_pf = false;  // the event's inputs were prefetched with its decision
     PHASE_INIT(et);
     while (go > 0) {
       if (!apply_first) {
         PHASE(et, 0, tid == 0);
         __syncthreads();
         PHASE(et, 1, tid == 0);
-        x = decide<NW>(sh, par, tinc, p.DelT, p.rX, nd);
+        {
+          const Best me = lane_entry<NW>(sh, par);
+          pf = engine_prefetch<NW>(d, sh, me, par, ns_pre);
+          x = decide_from<NW>(sh, me, par, tinc, p.DelT, p.rX, nd);
+          prefetch_landed(pf);
+        }
+        have_pf = true;
         if (x.stop) {
           fin = x.stop < 0 ? x.stop : 0;
           break;
@@ -1817,31 +1909,48 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         }
         susp = -1;
       } else {
-        // everything the event needs, in one round of independent loads: the captured
-        // voxel's neighbour table and pair ruqs, and the winning entry's capturing cell
-        // (per-slot rotation, centroid, voxel, grain) and shape factor
+        // the event's inputs: the captured voxel's neighbour table and pair ruqs, its cold
+        // flag and temperature, the winning entry's voxel, and the capturing cell's grain
+        // and rotation -- from the lanes that prefetched them with the decision (a resumed
+        // nucleation loads them here)
         const int cand = x.cand, sw = cand >= 0 ? cand / AMCD_NB : 0;
-        const int tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
-        const double rpair = (lane < AMCD_NB) ? d.aRN[(size_t)i1 * AMCD_NB + lane] : 0.0;
-        const int cold = d.aCold[i1];
-        const double T1 = d.aT[i1];
-        const int v1 = cand >= 0 ? d.cV[cand] : d.aVox[i1];  // (no entry: a resumed nucleation)
-        int vjs = 0, G = 1;
-        double fcap = 0.0, R[9], c0[3], cen[3], E0;
         // an entry of a list the set-up built carries its capture's re-centring
         const bool pre = !x.nuc && sw < ns_pre;
+        int tn, cold, v1, vjs = 0, G = 1;
+        double rpair, T1, fcap = 0.0, R[9], c0[3], cen[3], E0;
         double4 rc = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (pre) rc = d.cRC[cand];
-        if (!x.nuc) {
+        if (have_pf) {
+          const int wq = x.wave;
+          take_row(pf, wq, lane, tn, rpair);
+          cold = __builtin_amdgcn_readlane(pf.cold, wq);
+          T1 = lane_f64(pf.T1, wq);
+          v1 = __builtin_amdgcn_readlane(pf.v1, wq);
+          if (!x.nuc) {
 #pragma unroll
-          for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)sw + q];
-          if (!pre) {
-#pragma unroll
-            for (int q = 0; q < 3; ++q) c0[q] = d.gC[3 * (size_t)sw + q];
-            fcap = d.cF[cand];
-            vjs = d.gV[sw];
+            for (int q = 0; q < 9; ++q) R[q] = lane_f64(pf.R[q], wq);
+            G = __builtin_amdgcn_readlane(pf.G, wq);
+            if (pre)
+              rc = make_double4(lane_f64(pf.rc.x, wq), lane_f64(pf.rc.y, wq), lane_f64(pf.rc.z, wq),
+                                lane_f64(pf.rc.w, wq));
           }
-          G = d.gG[sw];
+        } else {
+          tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
+          rpair = (lane < AMCD_NB) ? d.aRN[(size_t)i1 * AMCD_NB + lane] : 0.0;
+          cold = d.aCold[i1];
+          T1 = d.aT[i1];
+          v1 = cand >= 0 ? d.cV[cand] : d.aVox[i1];  // (no entry: a resumed nucleation)
+          if (!x.nuc) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)sw + q];
+            G = d.gG[sw];
+            if (pre) rc = d.cRC[cand];
+          }
+        }
+        if (!x.nuc && !pre) {  // a cell made in this launch: re-centred below
+#pragma unroll
+          for (int q = 0; q < 3; ++q) c0[q] = d.gC[3 * (size_t)sw + q];
+          fcap = d.cF[cand];
+          vjs = d.gV[sw];
         }
         if (x.nuc) {  // S/VoxelsCA.C:486-506
           const double* P = d.pool + 9 * (size_t)pool_used;
