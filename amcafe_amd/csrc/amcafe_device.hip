@@ -1518,16 +1518,14 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
     // ---- APPLY
     const double Dt = x.dt;
     if (!x.nuc) {  // growth of every cell growing at this scan (S/VoxelsCA.C:511-516)
+      // straight-line over the K cells, so that their updates interleave
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        if (tgt1[k] < 0) continue;
+        const bool grows = tgt1[k] >= 0;
         const double e = E[k] + V[k] * Dt;
-        if (e < 0.0) {
-          E[k] = 0.0;
-          lo[k] = -fInf;  // clamped: recompute the bounds below
-        } else {
-          E[k] = e;
-        }
+        const bool clamp = grows & (e < 0.0);
+        E[k] = grows ? (clamp ? 0.0 : e) : E[k];
+        lo[k] = clamp ? -fInf : lo[k];  // clamped: recompute the bounds below
       }
     }
     T = T + Dt;  // the engine's tinc: same additions in the same order
@@ -1560,27 +1558,32 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
     ++nev;
     const bool all = nev >= (1 << 20);
     if (all) nev = 0;
+    bool any = false;
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const bool hit = tgt1[k] >= 0 && (tgt1[k] == i1 || lo[k] == -fInf || all);
-      if (!hit) continue;
-      const int s = mo + k * NO;
-      if (tgt1[k] == i1) {
-        const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
-        const SlotHead h = slot_head(d, dead, s, pos + 1, i1);
-        tgt1[k] = h.tgt1;
-        if constexpr (HC) {
-          c1[k] = (h.c1 - s * AMCD_NB) | ((((c1[k] >> 5) & ~31) + h.k1) << 5);
-          hc[s] = h.D1;
-          hc[HN + s] = h.Dn;
-          hc[2 * HN + s] = h.r1;
+    for (int k = 0; k < K; ++k) any |= (tgt1[k] >= 0) & ((tgt1[k] == i1) | (lo[k] == -fInf) | all);
+    if (__ballot(any) != 0) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const bool hit = tgt1[k] >= 0 && (tgt1[k] == i1 || lo[k] == -fInf || all);
+        if (!hit) continue;
+        const int s = mo + k * NO;
+        if (tgt1[k] == i1) {
+          const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
+          const SlotHead h = slot_head(d, dead, s, pos + 1, i1);
+          tgt1[k] = h.tgt1;
+          if constexpr (HC) {
+            c1[k] = (h.c1 - s * AMCD_NB) | ((((c1[k] >> 5) & ~31) + h.k1) << 5);
+            hc[s] = h.D1;
+            hc[HN + s] = h.Dn;
+            hc[2 * HN + s] = h.r1;
+          } else {
+            c1[k] = h.c1;
+          }
+          lo[k] = hi[k] = fInf;
+          if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
         } else {
-          c1[k] = h.c1;
+          tau_bounds(HC ? hc[s] : gmem(d.cD)[c1[k]], E[k], V[k], T, lo[k], hi[k]);
         }
-        lo[k] = hi[k] = fInf;
-        if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
-      } else {
-        tau_bounds(HC ? hc[s] : gmem(d.cD)[c1[k]], E[k], V[k], T, lo[k], hi[k]);
       }
     }
     skip = i1;
