@@ -227,6 +227,20 @@ __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, 
     const int kh0 = (int)floor((st.lz - depth) / p.dz - 0.5) - 1, kh1 = (int)floor(st.lz / p.dz - 0.5) + 1;
     int i, j, k;
     decode(d, v0, i, j, k);
+    // Nothing changes for a run of voxels in one plane outside [kh0, kh1] when none of them
+    // is hot (and no state byte carries bits the pass clears) and the plane lies on the
+    // same side of the previous and the current laser plane: T stays T0 or tS as before,
+    // and no state, grain or melt flag is written (below the laser plane the flag is
+    // cleared when T < T0, so tS < T0 rules the shortcut out there).
+    bool quiet = false;
+    if (valid && v0 + TV <= d.N && (k < kh0 || k > kh1) && (v0 + TV - 1) / d.nxy == k) {
+      unsigned busy = 0;
+#pragma unroll
+      for (int q = 0; q < TV; ++q) busy |= sv[q] & 0xbc;
+      const double zp = ((double)k + .5) * p.dz;
+      quiet = busy == 0 && (zp > st.lz) == (zp > lzp) && ((zp > st.lz) || !(p.tS < p.T0));
+    }
+    if (!quiet)
 #pragma unroll
     for (int q = 0; q < TV; ++q) {
       const int v = v0 + q;
@@ -439,49 +453,85 @@ __global__ __launch_bounds__(GB) void k_solid_to_mushy(Inst* I, const int* bstar
   const int b0 = (blockIdx.x - bstart[ii]) * SB;
   const int bend = min(d.N, b0 + SB);
   // liquid exists only in planes that held it at the last active-set pass or that the
-  // last temperature evaluation could melt: a block without such planes leaves at once
+  // last temperature evaluation could melt: a block with no such plane within one plane
+  // of its own leaves at once
   const int h0 = d.ls->zhot0, h1 = d.ls->zhot1;
   const int zb0 = b0 / d.nxy, zb1 = (bend - 1) / d.nxy;
   bool any = false;
-  for (int z = zb0; z <= zb1; ++z) any = any || (d.zliq[z] & 1u) || (z >= h0 && z <= h1);
+  for (int z = max(0, zb0 - 1); z <= min(nz - 1, zb1 + 1); ++z) any = any || (d.zliq[z] & 1u) || (z >= h0 && z <= h1);
   if (!any) return;
-  // each thread reads 16 consecutive state bytes at once and marks the solid Moore
-  // neighbours of the liquid ones mushy (the same voxels as asking every solid voxel
-  // whether a neighbour is liquid; concurrent marks store the same byte)
+  // A solid voxel with a liquid Moore neighbour turns mushy.  Each thread takes 16
+  // consecutive voxels; when they share a row it reads the 3x3 rows around them (16 bytes
+  // and the two end voxels each): the liquid voxels of each row, widened by one voxel to
+  // either side and or-ed over the nine rows, are the solid voxels of the run to mark.
+  // Liquid voxels are not changed by the pass, so every test reads settled states, and
+  // each thread writes its own voxels only.
   constexpr int PT = SB / GB;
   const int v0 = b0 + (int)threadIdx.x * PT;
   if (v0 >= bend) return;
-  uint8_t st[PT];
-  if (v0 + PT <= d.N) {
-    const uint4 w = *reinterpret_cast<const uint4*>(d.vs + v0);
-    const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+  int i, j, k;
+  decode(d, v0, i, j, k);
+  if (nx % PT == 0 && v0 + PT <= d.N) {
+    const uint4 c = *reinterpret_cast<const uint4*>(d.vs + v0);
+    const unsigned cw[4] = {c.x, c.y, c.z, c.w};
+    unsigned sol = 0;
 #pragma unroll
-    for (int q = 0; q < PT; ++q) st[q] = (uint8_t)(ws[q >> 2] >> (8 * (q & 3)));
-  } else {
-#pragma unroll
-    for (int q = 0; q < PT; ++q) st[q] = v0 + q < d.N ? d.vs[v0 + q] : 0;
-  }
-  bool liq_any = false;
-#pragma unroll
-  for (int q = 0; q < PT; ++q) liq_any = liq_any || (st[q] & 3) == 1;
-  if (!liq_any) return;
-  for (int q = 0; q < PT; ++q) {
-    if ((st[q] & 3) != 1) continue;
-    int i, j, k;
-    decode(d, v0 + q, i, j, k);
+    for (int q = 0; q < PT; ++q) sol |= (((cw[q >> 2] >> (8 * (q & 3))) & 3u) == 3u ? 1u : 0u) << q;
+    if (!sol) return;
+    unsigned near = 0;
     for (int dz = -1; dz <= 1; ++dz) {
       const int z = k + dz;
       if (z < 0 || z >= nz) continue;
       for (int dy = -1; dy <= 1; ++dy) {
         const int y = j + dy;
         if (y < 0 || y >= ny) continue;
-        uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * nx;
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int x = i + dx;
-          if (x < 0 || x >= nx) continue;
-          const uint8_t sn = row[x];
-          if ((sn & 3) == 3) row[x] = (sn & ~3) | 2;
+        const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * nx;
+        const uint4 w = *reinterpret_cast<const uint4*>(row + i);
+        const unsigned ww[4] = {w.x, w.y, w.z, w.w};
+        unsigned liq = 0;  // bit q + 1: voxel i + q of the row is liquid
+#pragma unroll
+        for (int q = 0; q < PT; ++q) liq |= (((ww[q >> 2] >> (8 * (q & 3))) & 3u) == 1u ? 1u : 0u) << (q + 1);
+        if (i > 0 && (row[i - 1] & 3) == 1) liq |= 1u;
+        if (i + PT < nx && (row[i + PT] & 3) == 1) liq |= 1u << (PT + 1);
+        near |= liq | (liq >> 1) | (liq >> 2);
+      }
+    }
+    const unsigned m = sol & near & ((1u << PT) - 1);
+    if (!m) return;
+    unsigned nw[4] = {cw[0], cw[1], cw[2], cw[3]};
+#pragma unroll
+    for (int q = 0; q < PT; ++q)
+      if ((m >> q) & 1u) nw[q >> 2] = (nw[q >> 2] & ~(3u << (8 * (q & 3)))) | (2u << (8 * (q & 3)));
+    *reinterpret_cast<uint4*>(d.vs + v0) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
+    return;
+  }
+  // rows that are not a multiple of 16 voxels, and the end of the domain: voxel by voxel
+  for (int q = 0; q < PT; ++q) {
+    const int v = v0 + q;
+    if (v >= bend) break;
+    const uint8_t sv = d.vs[v];
+    if ((sv & 3) == 3) {
+      bool hit = false;
+      for (int dz = -1; dz <= 1 && !hit; ++dz) {
+        const int z = k + dz;
+        if (z < 0 || z >= nz) continue;
+        for (int dy = -1; dy <= 1 && !hit; ++dy) {
+          const int y = j + dy;
+          if (y < 0 || y >= ny) continue;
+          const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * nx;
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int x = i + dx;
+            if (x >= 0 && x < nx && (row[x] & 3) == 1) hit = true;
+          }
         }
+      }
+      if (hit) d.vs[v] = (sv & ~3) | 2;
+    }
+    if (++i == nx) {
+      i = 0;
+      if (++j == ny) {
+        j = 0;
+        ++k;
       }
     }
   }
@@ -495,80 +545,165 @@ __global__ void k_zliq_clear(Inst* I, const amcd_step* S) {
 }
 // Active set: {mushy} U {liquid below tL} U {neighbours at or above tL of those}.
 // Writes bit 6 of vs and the per-block (ZB voxels) counts of active and mushy cells.
-__global__ __launch_bounds__(ZB) void k_active_flags(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
-  __shared__ int sA[ZB / 64], sM[ZB / 64];
+// A hot voxel that is neither mushy nor cool liquid is active when a Moore neighbour
+// is mushy or cool liquid.
+__device__ static bool hot_voxel_active(const Inst& d, int v) {
+  int i, j, k;
+  decode(d, v, i, j, k);
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = k + dz;
+    if (z < 0 || z >= d.p.nz) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = j + dy;
+      if (y < 0 || y >= d.p.ny) continue;
+      const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * d.p.nx;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int x = i + dx;
+        if (x < 0 || x >= d.p.nx || (dx == 0 && dy == 0 && dz == 0)) continue;
+        const uint8_t sn = row[x];
+        const int stn = sn & 3;
+        if (stn == 2 || (stn == 1 && !(sn & 0x80))) return true;
+      }
+    }
+  }
+  return false;
+}
+// One wave per block of ZB voxels (four per workgroup), 16 consecutive voxels per thread,
+// four state bytes at a time: per byte, bit 0 of lo/hi/hb is state bit 0, state bit 1
+// and the hot flag.
+__global__ __launch_bounds__(256) void k_active_flags(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
+  constexpr int PT = ZB / 64;
   int ii = find_inst(bstart, ninst, blockIdx.x);
   const Inst& d = I[ii];
   if (S[ii].skip) return;
-  int blk = blockIdx.x - bstart[ii];
-  int v = blk * ZB + threadIdx.x;
-  bool act = false, mushy = false;
-  if (v < d.N) {
-    uint8_t s = d.vs[v];
-    int st = s & 3;
-    bool hot = (s & 0x80) != 0;
-    mushy = st == 2;
-    act = mushy || (st == 1 && !hot);
-    if (!act && hot) {
-      int i, j, k;
-      decode(d, v, i, j, k);
-      for (int dz = -1; dz <= 1 && !act; ++dz) {
-        int z = k + dz;
-        if (z < 0 || z >= d.p.nz) continue;
-        for (int dy = -1; dy <= 1 && !act; ++dy) {
-          int y = j + dy;
-          if (y < 0 || y >= d.p.ny) continue;
-          const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * d.p.nx;
-          for (int dx = -1; dx <= 1; ++dx) {
-            int x = i + dx;
-            if (x < 0 || x >= d.p.nx || (dx == 0 && dy == 0 && dz == 0)) continue;
-            uint8_t sn = row[x];
-            int stn = sn & 3;
-            if (stn == 2 || (stn == 1 && !(sn & 0x80))) {
-              act = true;
-              break;
+  const int blk = (blockIdx.x - bstart[ii]) * 4 + (int)(threadIdx.x >> 6);
+  if (blk >= d.nblkZ) return;  // whole waves leave: nothing below synchronises the block
+  const int lane = threadIdx.x & 63;
+  const int v0 = blk * ZB + lane * PT;
+  int na = 0, nm = 0;
+  unsigned liq = 0;  // bit q: voxel v0 + q is liquid
+  if (v0 < d.N) {
+    const int nq = min(PT, d.N - v0);
+    unsigned w[4] = {0, 0, 0, 0};
+    if (nq == PT) {
+      const uint4 x = *reinterpret_cast<const uint4*>(d.vs + v0);
+      w[0] = x.x;
+      w[1] = x.y;
+      w[2] = x.z;
+      w[3] = x.w;
+    } else {
+      for (int q = 0; q < nq; ++q) w[q >> 2] |= (unsigned)d.vs[v0 + q] << (8 * (q & 3));
+    }
+    const unsigned valid = nq == PT ? 0xffffu : (1u << nq) - 1;
+    // hot voxels that are neither mushy nor cool liquid ask their Moore neighbours; when
+    // the 16 voxels share a row, the 3x3 rows around them are read whole: their mushy or
+    // cool liquid voxels, widened by one voxel either way and or-ed over the nine rows
+    unsigned hotpend = 0;  // bit q: voxel v0 + q is hot and not active by itself
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const unsigned x = w[r];
+      const unsigned lo = x & 0x01010101u, hi = (x >> 1) & 0x01010101u, hb = (x >> 7) & 0x01010101u;
+      const unsigned self = (hi & ~lo) | (lo & ~hi & ~hb);
+      const unsigned pend = hb & ~self;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) hotpend |= ((pend >> (8 * q)) & 1u) << (4 * r + q);
+    }
+    hotpend &= valid;
+    unsigned nearA = 0;  // bit q: a Moore neighbour of voxel v0 + q is mushy or cool liquid
+    if (hotpend) {
+      if (d.p.nx % PT == 0 && nq == PT) {
+        int i, j, k;
+        decode(d, v0, i, j, k);
+        for (int dz = -1; dz <= 1; ++dz) {
+          const int z = k + dz;
+          if (z < 0 || z >= d.p.nz) continue;
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int y = j + dy;
+            if (y < 0 || y >= d.p.ny) continue;
+            const uint8_t* row = d.vs + (size_t)z * d.nxy + (size_t)y * d.p.nx;
+            const uint4 u = *reinterpret_cast<const uint4*>(row + i);
+            const unsigned uw[4] = {u.x, u.y, u.z, u.w};
+            unsigned am = 0;  // bit q + 1: voxel i + q of the row is mushy or cool liquid
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const unsigned x = uw[r];
+              const unsigned lo = x & 0x01010101u, hi = (x >> 1) & 0x01010101u, hb = (x >> 7) & 0x01010101u;
+              const unsigned a4 = (hi & ~lo) | (lo & ~hi & ~hb);
+#pragma unroll
+              for (int q = 0; q < 4; ++q) am |= ((a4 >> (8 * q)) & 1u) << (4 * r + q + 1);
             }
+            auto edge = [](uint8_t sn) { const int st = sn & 3; return st == 2 || (st == 1 && !(sn & 0x80)); };
+            if (i > 0 && edge(row[i - 1])) am |= 1u;
+            if (i + PT < d.p.nx && edge(row[i + PT])) am |= 1u << (PT + 1);
+            nearA |= am | (am >> 1) | (am >> 2);
           }
+        }
+      } else {
+        for (unsigned m = hotpend; m; m &= m - 1) {
+          const int q = __builtin_ctz(m);
+          if (hot_voxel_active(d, v0 + q)) nearA |= 1u << q;
         }
       }
     }
-    const uint8_t s1 = (s & ~0x40) | (act ? 0x40 : 0);
-    if (s1 != s) d.vs[v] = s1;
-  }
-  // planes holding liquid now (for the next step's solid->mushy pass): at most one atomic
-  // per wave and plane, and none once the flag is seen set (every liquid run of a plane
-  // would otherwise hit the same word)
-  {
-    const bool liq = v < d.N && (d.vs[v] & 3) == 1;
-    const int kz = v < d.N ? v / d.nxy : 0;
-    const int lane = threadIdx.x & 63;
-    bool first;
-    if (d.nxy >= 64) {  // a wave spans at most two planes
-      const unsigned long long bl = __ballot(liq), same = __ballot(kz == __shfl(kz, 0));
-      const unsigned long long g1 = bl & same, g2 = bl & ~same;
-      first = (g1 && lane == __ffsll((long long)g1) - 1) || (g2 && lane == __ffsll((long long)g2) - 1);
-    } else {
-      const int kprev = __shfl_up(kz, 1);
-      const bool lprev = __shfl_up((int)liq, 1) != 0;
-      first = liq && (lane == 0 || !lprev || kprev != kz);
+    unsigned nw[4];
+    bool changed = false;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const unsigned x = w[r];
+      const unsigned lo = x & 0x01010101u, hi = (x >> 1) & 0x01010101u, hb = (x >> 7) & 0x01010101u;
+      unsigned vb = 0, nb = 0;  // bytes of this word inside the domain; hot bytes with an active neighbour
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        vb |= ((valid >> (4 * r + q)) & 1u) << (8 * q);
+        nb |= (((hotpend & nearA) >> (4 * r + q)) & 1u) << (8 * q);
+      }
+      const unsigned mu = hi & ~lo & vb, lq = lo & ~hi & vb;
+      const unsigned ac = mu | (lq & ~hb) | nb;
+      nw[r] = (x & ~0x40404040u) | (ac << 6);
+      changed = changed || nw[r] != x;
+      na += __popc(ac);
+      nm += __popc(mu);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) liq |= ((lq >> (8 * q)) & 1u) << (4 * r + q);
     }
-    if (first && !(d.zliq[kz] & 1u)) atomicOr(&d.zliq[kz], 1u);
-  }
-  unsigned long long ba = __ballot(act), bm = __ballot(mushy);
-  int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sA[w] = __popcll(ba);
-    sM[w] = __popcll(bm);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int a = 0, m = 0;
-    for (int q = 0; q < ZB / 64; ++q) {
-      a += sA[q];
-      m += sM[q];
+    if (changed) {
+      if (nq == PT) {
+        *reinterpret_cast<uint4*>(d.vs + v0) = make_uint4(nw[0], nw[1], nw[2], nw[3]);
+      } else {
+        for (int q = 0; q < nq; ++q) d.vs[v0 + q] = (uint8_t)(nw[q >> 2] >> (8 * (q & 3)));
+      }
     }
-    d.bcA[blk] = a;
-    d.bcM[blk] = m;
+  }
+  // planes holding liquid now (for the next step's solid->mushy pass): one atomic per
+  // wave and plane, and none once the flag is seen set.  A run of 16 spans at most two
+  // planes unless a plane holds fewer than 16 voxels.
+  int lp0 = -1, lp1 = -1;
+  if (liq) {
+    lp0 = (v0 + __builtin_ctz(liq)) / d.nxy;
+    lp1 = (v0 + 31 - __builtin_clz(liq)) / d.nxy;
+    if (d.nxy < PT)
+      for (unsigned m = liq; m; m &= m - 1) {
+        const int kz = (v0 + __builtin_ctz(m)) / d.nxy;
+        if (!(d.zliq[kz] & 1u)) atomicOr(&d.zliq[kz], 1u);
+      }
+  }
+  for (int pass = 0; pass < 2; ++pass) {
+    const int pz = pass ? lp1 : lp0;
+    unsigned long long pend = __ballot(pz >= 0);
+    while (pend) {
+      const int leader = __builtin_ctzll(pend);
+      const int zl = __builtin_amdgcn_readlane(pz, leader);
+      pend &= ~__ballot(pz == zl);
+      if (lane == leader && !(d.zliq[zl] & 1u)) atomicOr(&d.zliq[zl], 1u);
+    }
+  }
+  for (int o = 32; o >= 1; o >>= 1) {
+    na += __shfl_xor(na, o);
+    nm += __shfl_xor(nm, o);
+  }
+  if (lane == 0) {
+    d.bcA[blk] = na;
+    d.bcM[blk] = nm;
   }
 }
 
@@ -674,49 +809,88 @@ __global__ __launch_bounds__(1024) void k_scan(Inst* I, const amcd_step* S, amcd
 // gathers of the per-cell state, the scan position of mushy cells and the uniform
 // draws ruq (S/VoxelsCA.C:380): the engine is seeded with 30*tInd+seed1 and active
 // cell a takes engine outputs 2a+1 and 2a+2 (jump-ahead 16807^(2a)).
-__global__ __launch_bounds__(ZB) void k_compact(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
-  __shared__ int sA[ZB / 64], sM[ZB / 64];
+__global__ __launch_bounds__(256) void k_compact(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
+  // one wave per block of ZB voxels (four per workgroup), 16 consecutive voxels per thread
+  constexpr int PT = ZB / 64;
   int ii = find_inst(bstart, ninst, blockIdx.x);
   const Inst& d = I[ii];
   if (d.ls->done) return;
-  int blk = blockIdx.x - bstart[ii];
-  int v = blk * ZB + threadIdx.x;
-  bool act = false, mushy = false;
-  uint8_t s = 0;
-  if (v < d.N) {
-    s = d.vs[v];
-    act = (s & 0x40) != 0;
-    mushy = (s & 3) == 2;
+  const int blk = (blockIdx.x - bstart[ii]) * 4 + (int)(threadIdx.x >> 6);
+  if (blk >= d.nblkZ) return;
+  // a block without active cells (its global offset equals the next block's) has nothing
+  // to write; most of the domain is such blocks
+  const int o0 = d.bcA[blk] + d.chA[blk >> 10];
+  const int o1 = blk + 1 < d.nblkZ ? d.bcA[blk + 1] + d.chA[(blk + 1) >> 10] : d.ls->Na;
+  if (o1 == o0) return;
+  const int lane = threadIdx.x & 63;
+  const int v0 = blk * ZB + lane * PT;
+  unsigned am = 0, mm = 0;  // bit q: voxel v0 + q is active / mushy
+  if (v0 < d.N) {
+    const int nq = min(PT, d.N - v0);
+    if (nq == PT) {
+      const uint4 x = *reinterpret_cast<const uint4*>(d.vs + v0);
+      const unsigned w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+      for (int q = 0; q < PT; ++q) {
+        const unsigned b = (w[q >> 2] >> (8 * (q & 3))) & 0xffu;
+        am |= ((b >> 6) & 1u) << q;
+        mm |= ((b & 3u) == 2u ? 1u : 0u) << q;
+      }
+    } else {
+      for (int q = 0; q < nq; ++q) {
+        const unsigned b = d.vs[v0 + q];
+        am |= ((b >> 6) & 1u) << q;
+        mm |= ((b & 3u) == 2u ? 1u : 0u) << q;
+      }
+    }
   }
-  unsigned long long ba = __ballot(act), bm = __ballot(mushy);
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  if (lane == 0) {
-    sA[w] = __popcll(ba);
-    sM[w] = __popcll(bm);
+  // exclusive prefix over the wave's lanes of the active and mushy counts (voxel order)
+  const int ca = __popc(am), cm = __popc(mm);  // (mushy voxels are active)
+  int ia = ca, im = cm;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int ta = __shfl_up(ia, o), tm = __shfl_up(im, o);
+    if (lane >= o) {
+      ia += ta;
+      im += tm;
+    }
   }
-  __syncthreads();
-  if (!act) return;
-  int offA = d.bcA[blk] + d.chA[blk >> 10], offM = d.bcM[blk] + d.chM[blk >> 10];
-  for (int q = 0; q < w; ++q) {
-    offA += sA[q];
-    offM += sM[q];
+  const int ea = ia - ca, em = im - cm;  // this lane's exclusive prefixes
+  const int total = __shfl(ia, 63);
+  const int m0 = d.bcM[blk] + d.chM[blk >> 10];
+  const unsigned int x0 = lcg_seed(30ull * (unsigned long long)S[ii].tInd + d.p.seed1);
+  // the block's active voxels, 64 at a time: lane L takes the (base + L)-th of them, found
+  // by a binary search over the lanes' prefixes and a walk over that lane's mask
+  for (int base = 0; base < total; base += 64) {
+    const int idx = base + lane;
+    int t = 0;  // the lane holding voxel idx: the last lane whose prefix is <= idx
+    for (int step = 32; step >= 1; step >>= 1) {
+      const int c = t + step;
+      const int ec = __shfl(ea, c < 64 ? c : 63);
+      if (c < 64 && ec <= idx) t = c;
+    }
+    const unsigned mt = (unsigned)__shfl((int)am, t), mmt = (unsigned)__shfl((int)mm, t);
+    const int et = __shfl(ea, t), emt = __shfl(em, t);
+    if (idx >= total) continue;
+    unsigned m = mt;
+    for (int r = idx - et; r > 0; --r) m &= m - 1;
+    const int q = __builtin_ctz(m);
+    const int v = blk * ZB + t * PT + q;
+    const int a = o0 + idx;
+    const bool mushy = (mmt >> q) & 1u;
+    d.aVox[a] = v;
+    d.amap[v] = a;
+    d.aVS[a] = d.vs[v] & 3;
+    d.aG[a] = d.gid[v];
+    d.aT[a] = d.T[v];
+    d.aExt[a] = d.ext[v];
+    d.aCx[a] = d.cx[v];
+    d.aCy[a] = d.cy[v];
+    d.aCz[a] = d.cz[v];
+    d.aScan[a] = mushy ? m0 + emt + __popc(mmt & ((1u << q) - 1)) : -1;
+    const unsigned int x1 = mod_m((unsigned long long)x0 * lcg_pow(2ull * a + 1));
+    const unsigned int x2 = lcg_next(x1);
+    d.ruq[a] = (canonical(x1, x2) - .5) * d.p.dx / 100.;
   }
-  int a = offA + __popcll(ba & lt);
-  d.aVox[a] = v;
-  d.amap[v] = a;
-  d.aVS[a] = s & 3;
-  d.aG[a] = d.gid[v];
-  d.aT[a] = d.T[v];
-  d.aExt[a] = d.ext[v];
-  d.aCx[a] = d.cx[v];
-  d.aCy[a] = d.cy[v];
-  d.aCz[a] = d.cz[v];
-  d.aScan[a] = mushy ? offM + __popcll(bm & lt) : -1;
-  unsigned int x0 = lcg_seed(30ull * (unsigned long long)S[ii].tInd + d.p.seed1);
-  unsigned int x1 = mod_m((unsigned long long)x0 * lcg_pow(2ull * a + 1));
-  unsigned int x2 = lcg_next(x1);
-  d.ruq[a] = (canonical(x1, x2) - .5) * d.p.dx / 100.;
 }
 
 // Voxel centre, S/VoxelsCA.C:409-414.
@@ -2320,8 +2494,8 @@ struct amcd_batch {
   std::vector<Inst> h;      // host mirror of the descriptors
   Inst* d_inst = nullptr;
   int *d_bstartG = nullptr, *d_bstartZ = nullptr, *d_astart = nullptr, *d_bstartS = nullptr, *d_bstartT = nullptr,
-      *d_lstart = nullptr, *d_cstart = nullptr;
-  std::vector<int> bstartG, bstartZ, astart, bstartS, bstartT, lstart, cstart;
+      *d_lstart = nullptr, *d_cstart = nullptr, *d_bstartF = nullptr;
+  std::vector<int> bstartG, bstartZ, astart, bstartS, bstartT, lstart, cstart, bstartF;
   amcd_step* d_steps = nullptr;
   amcd_out* d_outs = nullptr;
   amcd_step* h_steps = nullptr;  // pinned
@@ -2414,6 +2588,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   b->lstart.assign(n + 1, 0);
   b->cstart.assign(n + 1, 0);
   b->bstartZ.assign(n + 1, 0);
+  b->bstartF.assign(n + 1, 0);
   b->astart.assign(n + 1, 0);
   for (int i = 0; i < n; ++i) {
     Inst& d = b->h[i];
@@ -2477,11 +2652,13 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     b->bstartT[i + 1] = b->bstartT[i] + d.nblkT;
     b->cstart[i + 1] = b->cstart[i] + (d.nblkZ + 1023) / 1024;
     b->bstartZ[i + 1] = b->bstartZ[i] + d.nblkZ;
+    b->bstartF[i + 1] = b->bstartF[i] + (d.nblkZ + 3) / 4;
   }
   if (dalloc(b, &b->d_inst, (size_t)n) || dalloc(b, &b->d_bstartG, (size_t)n + 1) ||
       dalloc(b, &b->d_bstartS, (size_t)n + 1) || dalloc(b, &b->d_bstartT, (size_t)n + 1) ||
       dalloc(b, &b->d_lstart, (size_t)n + 1) || dalloc(b, &b->d_cstart, (size_t)n + 1) ||
       dalloc(b, &b->d_bstartZ, (size_t)n + 1) || dalloc(b, &b->d_astart, (size_t)n + 1) ||
+      dalloc(b, &b->d_bstartF, (size_t)n + 1) ||
       dalloc(b, &b->d_steps, (size_t)n) || dalloc(b, &b->d_outs, (size_t)n)) {
     snprintf(err, errlen, "%s", g_dev_err.c_str());
     amcd_destroy(b);
@@ -2495,7 +2672,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   }
   const std::pair<int*, const std::vector<int>*> starts[] = {{b->d_bstartG, &b->bstartG}, {b->d_bstartS, &b->bstartS},
                                                              {b->d_bstartT, &b->bstartT}, {b->d_cstart, &b->cstart},
-                                                             {b->d_bstartZ, &b->bstartZ}};
+                                                             {b->d_bstartZ, &b->bstartZ}, {b->d_bstartF, &b->bstartF}};
   for (const auto& st : starts)
     if (hipMemcpyAsync(st.first, st.second->data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream) !=
         hipSuccess) {
@@ -2718,11 +2895,11 @@ static int launch_update(amcd_batch* b, bool fused_temperature) {
                      b->d_steps);
   tmark(b, 2);
   hipLaunchKernelGGL(k_zliq_clear, dim3(n), dim3(256), 0, b->stream, b->d_inst, b->d_steps);
-  hipLaunchKernelGGL(k_active_flags, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
+  hipLaunchKernelGGL(k_active_flags, dim3(b->bstartF[n]), dim3(256), 0, b->stream, b->d_inst, b->d_bstartF, n,
                      b->d_steps);
   hipLaunchKernelGGL(k_scan_chunks, dim3(b->cstart[n]), dim3(1024), 0, b->stream, b->d_inst, b->d_cstart, n);
   hipLaunchKernelGGL(k_scan, dim3(n), dim3(1024), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
-  hipLaunchKernelGGL(k_compact, dim3(b->bstartZ[n]), dim3(ZB), 0, b->stream, b->d_inst, b->d_bstartZ, n,
+  hipLaunchKernelGGL(k_compact, dim3(b->bstartF[n]), dim3(256), 0, b->stream, b->d_inst, b->d_bstartF, n,
                      b->d_steps);
   hipLaunchKernelGGL(k_loop_rng, dim3(n), dim3(64), 0, b->stream, b->d_inst, b->d_steps);
   tmark(b, 3);
