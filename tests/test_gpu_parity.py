@@ -76,6 +76,14 @@ def test_small_deck_full_run():
     assert n > 200
 
 
+def test_odd_sized_deck_full_run():
+    """37x29x20 voxels: rows and the voxel count are not multiples of 16, so the grid
+    passes that work in runs of 16 voxels take their voxel-by-voxel row paths and their
+    partial last run."""
+    n = _lockstep("t_odd.in", 10**6)
+    assert n > 200
+
+
 def test_c0_single_track_full_run():
     """Workload c0 (128x64x32, one track, one layer): every step bit-identical."""
     n = _lockstep("c0_single_track.in", 10**6)
