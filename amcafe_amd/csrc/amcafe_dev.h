@@ -64,6 +64,7 @@ typedef struct {
   int err_nonanalytic; // a growing cell had a temperature with no host-computed velocity
   long long slot_visits;  // growing cells with live candidates, summed over the loop's scans
   long long cand_evals;   // capture times evaluated, summed over the loop's scans
+  long long interface_events;  // growing-list length summed over the loop's events
 } amcd_out;
 
 typedef struct amcd_batch amcd_batch;

@@ -2020,7 +2020,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
       owner_loop_mem<NT>(d, sh, s_dead, go, ls->tinc, visits, evals);
   } else {
     // engine state, identical in every lane of wave 0
-    long long e_visits = 0, e_evals = 0;
+    long long e_visits = 0, e_evals = 0, e_units = 0;
     double tinc = ls->tinc;
     int vSind2 = ls->vSind2, nGrain = ls->nGrain, pool_used = ls->pool_used;
     int events = ls->events, captures = ls->captures, nucs = ls->nucs;
@@ -2065,6 +2065,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           break;
         }
         events++;
+        e_units += nslots;  // the growing list this event's decision covered
         if ((nd & 63) == 0) {  // first draw of batch nd/64: write batch nd/64 + 1
           sh.ubuf[((nd >> 6) + 1) & 1][lane] = draw_batch(x0, cb, aj);
           cb = mod_m((unsigned long long)cb * c128);
@@ -2277,6 +2278,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     }
     visits += e_visits;
     evals += e_evals;
+    if (tid == 0) atomicAdd((unsigned long long*)&O[blockIdx.x].interface_events, (unsigned long long)e_units);
   }
   __syncthreads();
   // ---- loop state and counters

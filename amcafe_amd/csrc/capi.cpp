@@ -142,6 +142,7 @@ static void to_stats(const amcd_out& o, amcafe_stats* s) {
   s->tinc = o.tinc;
   s->slot_visits = o.slot_visits;
   s->cand_evals = o.cand_evals;
+  s->interface_events = o.interface_events;
 }
 
 // ====================================================================================

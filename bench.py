@@ -51,9 +51,11 @@ def algorithmic_bytes(stage, nvox, st):
         return 12.0 * nvox
     if stage == 1:   # vState read + conditional write
         return 2.0 * nvox
-    if stage == 4:   # capture loop: per live growing cell per scan ExtA r/w 16, vhat 8, scan
-        # key + state 12; per live candidate D 8, ruqs 8, target 4 + its state 1
-        return 36.0 * st["slot_visits"] + 21.0 * st["cand_evals"]
+    if stage == 4:   # capture loop, per growing-interface cell per event (the reference scans every
+        # growing cell at every event, S/VoxelsCA.C:386-440): ExtA r/w 16, vhat 8, ruq 8, state and
+        # grain 4, one cached candidate distance 8.  The cells stay on chip here, so this is the
+        # algorithmic figure; the HBM bytes actually moved are `traffic`.
+        return 44.0 * st["interface_events"]
     return None
 
 
@@ -163,7 +165,7 @@ def main():
                 sim.info(0))
     sim.set_timing(True)
     stage_ms = np.zeros(6)
-    tot = dict(slot_visits=0, cand_evals=0, n_events=0, n_active=0, n_growing=0)
+    tot = dict(slot_visits=0, cand_evals=0, interface_events=0, n_events=0, n_active=0, n_growing=0)
     steps = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):

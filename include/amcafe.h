@@ -77,6 +77,8 @@ typedef struct {
   double tinc;         /* time advanced by the capture loop */
   long long slot_visits; /* growing cells with live candidates, summed over the loop's scans */
   long long cand_evals;  /* capture times evaluated, summed over the loop's scans */
+  long long interface_events; /* growing cells summed over the capture loop's events (the
+                                 cells the reference scans per event, S/VoxelsCA.C:386-400) */
 } amcafe_stats;
 
 int amcafe_domain_create(const amcafe_domain_desc* desc, amcafe_domain** out);
