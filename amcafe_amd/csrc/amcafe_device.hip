@@ -2507,6 +2507,8 @@ struct amcd_batch {
   double last_ms[16] = {0};
   bool timing = false;
   int tier = 1;  // capture-loop capacity tier (see launch_event_loop)
+  int min_tier = 0;  // lowest tier a step starts in (AMCAFE_CAPTURE_TIER_MIN; tests use it to
+                     // run small decks in the larger tiers)
 };
 
 static thread_local std::string g_dev_err;
@@ -2573,6 +2575,10 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   amcd_batch* b = new amcd_batch();
   b->n = n;
   b->h.resize(n);
+  if (const char* mt = getenv("AMCAFE_CAPTURE_TIER_MIN")) {
+    b->min_tier = std::max(0, std::min(4, atoi(mt)));
+    b->tier = std::max(b->tier, b->min_tier);
+  }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     snprintf(err, errlen, "hipStreamCreate failed");
     delete b;
@@ -3048,7 +3054,7 @@ static void pick_tier(amcd_batch* b, const amcd_out* outs) {
   for (int i = 0; i < b->n; ++i) mx = std::max(mx, outs[i].n_slots);
   int t = 0;
   while (t < kNTiers - 1 && kTierCap[t] < mx + mx / 32) ++t;
-  b->tier = t;
+  b->tier = std::max(t, b->min_tier);
 }
 
 static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, bool fused) {

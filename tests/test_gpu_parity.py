@@ -84,6 +84,18 @@ def test_odd_sized_deck_full_run():
     assert n > 200
 
 
+@pytest.mark.parametrize("deck,tier", [("t_small.in", 2), ("t_small.in", 3), ("t_small.in", 4),
+                                       ("c0_single_track.in", 4)])
+def test_capture_tiers_full_run(monkeypatch, deck, tier):
+    """Small decks run with the capture loop held in a larger tier (AMCAFE_CAPTURE_TIER_MIN):
+    2 = eight cells per owner thread with the LDS head cache, 3 = sixteen cells per thread
+    reading heads from the lists, 4 = every cell's state in memory.  Every step against the
+    oracle."""
+    monkeypatch.setenv("AMCAFE_CAPTURE_TIER_MIN", str(tier))
+    n = _lockstep(deck, 10**6)
+    assert n > 100
+
+
 def test_c0_single_track_full_run():
     """Workload c0 (128x64x32, one track, one layer): every step bit-identical."""
     n = _lockstep("c0_single_track.in", 10**6)
