@@ -1794,12 +1794,35 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
   const float fInf = __builtin_inff();
   const amcd_params& p = d.p;
   int ns = go != -2 ? sh.nslots : 0;  // slots this lane knows of (new ones arrive by mailbox)
+  // The lane's smallest upper bound and its two smallest lower bounds (with their slots)
+  // and third smallest lower bound, gathered whenever every cell's bounds pass through the
+  // lane (set-up, and the growth pass of each event): the next evaluation needs no pass
+  // over the cells unless a third cell reaches the wave's smallest upper bound.
+  float hmin = fInf, l0 = fInf, l1 = fInf, l2 = fInf;
+  int f0 = -1, f1 = -1;
+  auto note = [&](int s, float2 lh) {
+    hmin = fminf(hmin, lh.y);
+    if (lh.x < l0) {
+      l2 = l1;
+      l1 = l0;
+      f1 = f0;
+      l0 = lh.x;
+      f0 = s;
+    } else if (lh.x < l1) {
+      l2 = l1;
+      l1 = lh.x;
+      f1 = s;
+    } else if (lh.x < l2) {
+      l2 = lh.x;
+    }
+  };
   for (int s = me; s < ns; s += NO) {
     const SlotHead h = slot_head(d, dead, s, 0, -1);
     float lo = fInf, hi = fInf;
     if (h.tgt1 >= 0) tau_bounds(h.D1, d.gExt[s], d.gVhat[s], T, lo, hi);
     d.gHd[s] = make_int2(h.c1, h.tgt1);
     d.gLH[s] = make_float2(lo, hi);
+    note(s, make_float2(lo, hi));
   }
   int skip = -1;
   int par = 0;
@@ -1817,65 +1840,63 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
   while (go > 0) {
     if (!apply_first) {
       // ---- EVAL: the wave's smallest upper bound, then the cells whose lower bound reaches it
-      float u = fInf, lmin = fInf;
-      for (int s0 = me; s0 < ns; s0 += MB * NO) {
-        float2 lh[MB];
-#pragma unroll
-        for (int q = 0; q < MB; ++q) {
-          const int s = s0 + q * NO;
-          lh[q] = s < ns ? d.gLH[s] : make_float2(fInf, fInf);
-        }
-#pragma unroll
-        for (int q = 0; q < MB; ++q) {
-          u = fminf(u, lh[q].y);
-          lmin = fminf(lmin, lh[q].x);
-        }
-      }
-      u = wave_min(u);
+      const float u = wave_min(hmin);
       Best b = best_none();
-      if (lmin <= u) {
-        for (int s = me; s < ns; s += NO) {
-          const float lo = d.gLH[s].x;
-          if (!(lo != fInf && lo <= u)) continue;
-          const int2 hd = d.gHd[s];
-          const int c = hd.x;
-          const int pos = c - s * AMCD_NB;
-          const double e = d.gExt[s], v = d.gVhat[s];
-          const double D1 = d.cD[c];
-          const double Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
-          const double rr = d.cR[c];
-          const int kb = d.gScan[s] * 32;
-          const int kk = d.cK[c];
-          visits++;
-          evals++;
-          const double t1 = (D1 - e) / v;
-          if (t1 <= 1e6) {
-            Best y;
-            y.t = t1;
-            y.r = rr;
-            y.key = kb + kk;
-            y.cand = c;
-            y.tgt = hd.y;
-            if ((Dn - e) / v == t1) {
-              const int n = d.gN[s];
-              for (int q = pos + 1; q < n; ++q) {
-                const int tq = d.cTgt[c - pos + q];
-                if (!live_target(dead, tq, skip)) continue;
-                evals++;
-                const double tt = (d.cD[c - pos + q] - e) / v;
-                if (!(tt == t1)) break;
-                Best z;
-                z.t = tt;
-                z.r = d.cR[c - pos + q];
-                z.key = kb + d.cK[c - pos + q];
-                z.cand = c - pos + q;
-                z.tgt = tq;
-                if (better(z, y)) y = z;
-              }
+      auto evaluate = [&](int s) {  // exact capture time of slot s's head (S/VoxelsCA.C:420-440)
+        const int2 hd = d.gHd[s];
+        const int c = hd.x;
+        const int pos = c - s * AMCD_NB;
+        const double e = d.gExt[s], v = d.gVhat[s];
+        const double D1 = d.cD[c];
+        const double Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
+        const double rr = d.cR[c];
+        const int kb = d.gScan[s] * 32;
+        const int kk = d.cK[c];
+        visits++;
+        evals++;
+        const double t1 = (D1 - e) / v;
+        if (t1 <= 1e6) {
+          Best y;
+          y.t = t1;
+          y.r = rr;
+          y.key = kb + kk;
+          y.cand = c;
+          y.tgt = hd.y;
+          if ((Dn - e) / v == t1) {
+            const int n = d.gN[s];
+            for (int q = pos + 1; q < n; ++q) {
+              const int tq = d.cTgt[c - pos + q];
+              if (!live_target(dead, tq, skip)) continue;
+              evals++;
+              const double tt = (d.cD[c - pos + q] - e) / v;
+              if (!(tt == t1)) break;
+              Best z;
+              z.t = tt;
+              z.r = d.cR[c - pos + q];
+              z.key = kb + d.cK[c - pos + q];
+              z.cand = c - pos + q;
+              z.tgt = tq;
+              if (better(z, y)) y = z;
             }
-            if (better(y, b)) b = y;
           }
+          if (better(y, b)) b = y;
         }
+      };
+      if (l2 <= u) {  // a third cell may reach u: every cell of the lane, MB at a time
+        for (int s0 = me; s0 < ns; s0 += MB * NO) {
+          float lob[MB];
+#pragma unroll
+          for (int q = 0; q < MB; ++q) {
+            const int s = s0 + q * NO;
+            lob[q] = s < ns ? d.gLH[s].x : fInf;
+          }
+#pragma unroll
+          for (int q = 0; q < MB; ++q)
+            if (lob[q] != fInf && lob[q] <= u) evaluate(s0 + q * NO);
+        }
+      } else {
+        if (f0 >= 0 && l0 != fInf && l0 <= u) evaluate(f0);
+        if (f1 >= 0 && l1 != fInf && l1 <= u) evaluate(f1);
       }
       b = wave_best(b);
       if (lane == 0) sh.sb[par][w] = b;
@@ -1902,6 +1923,8 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
     ++nev;
     const bool all = nev >= (1 << 20);
     if (all) nev = 0;
+    hmin = l0 = l1 = l2 = fInf;
+    f0 = f1 = -1;
     for (int s0 = me; s0 < ns; s0 += MB * NO) {
       int2 hdb[MB];
       float2 lhb[MB];
@@ -1944,6 +1967,7 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
           tau_bounds(d.cD[hd.x], e, v, T, lh.x, lh.y);
           d.gLH[s] = lh;
         }
+        note(s, lh);
       }
     }
     skip = i1;
