@@ -58,7 +58,15 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
   return (__attribute__((address_space(1))) T*)p;
 }
 
-#define ZB 1024     // voxels per compaction block (one per thread)
+// lane l's value of v (l must be the same in every lane)
+__device__ static inline double lane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+#define ZB 1024     // voxels per compaction block (one wave, 16 per thread)
 #define GB 256      // threads per grid-wide block
 #define SB 4096     // voxels per solid->mushy block (GB threads, 16 voxels each)
 
@@ -1124,10 +1132,11 @@ __device__ static inline void grow_list(const Inst& d, int slot, int lane) {
     rc = make_double4(nc[0], nc[1], nc[2], ne);
   }
   const unsigned long long bal = __ballot(liq);
-  int rank = 0;
+  int rank = 0;  // (lane j's entry read with readlane: j is a constant of the unrolled loop)
+#pragma unroll
   for (int j = 0; j < AMCD_NB; ++j) {
-    const double Dj = __shfl(D, j), rj = __shfl(r, j);
-    const int kj = __shfl(key, j);
+    const double Dj = lane_f64(D, j), rj = lane_f64(r, j);
+    const int kj = __builtin_amdgcn_readlane(key, j);
     if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
   }
   const size_t cb = (size_t)slot * AMCD_NB;
@@ -1195,12 +1204,6 @@ __device__ static inline Best best_none() {
 __device__ static inline bool better(const Best& a, const Best& b) {
   const bool tl = a.t < b.t, tg = a.t > b.t, rl = a.r < b.r, rg = a.r > b.r;
   return tl | (!tg & (rl | (!rg & (a.key < b.key))));
-}
-__device__ static inline double lane_f64(double v, int l) {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 // Wave-wide reductions by DPP row operations: two quad permutes, the half-row and row
 // mirrors leave a row's result in all 16 of its lanes, row_bcast:15 and row_bcast:31
@@ -2214,9 +2217,10 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             key = scan * 32 + lane;
           }
           int rank = 0;  // position in (D, ruqs, key) order
+#pragma unroll
           for (int j = 0; j < AMCD_NB; ++j) {
-            const double Dj = __shfl(D, j), rj = __shfl(r, j);
-            const int kj = __shfl(key, j);
+            const double Dj = lane_f64(D, j), rj = lane_f64(r, j);
+            const int kj = __builtin_amdgcn_readlane(key, j);
             if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
           }
           if (liq) {
