@@ -3,19 +3,27 @@
 //
 // One step of one or many independent domains ("instances") runs as:
 //   k_temperature      TempField::AnalyticTempCurr (S/TempField.C:139-217), optionally
-//                      fused with SetLiquid3 (S/VoxelsCA.C:1098-1116)
+//                      fused with SetLiquid3 (S/VoxelsCA.C:1098-1116); k_temp_commit
+//                      records the laser plane the field now matches.  (k_easm: the
+//                      EASM temperature, S/TempField.C:79-137, when asked for.)
 //   k_setliquid        SetLiquid3 alone, when the temperature was a separate call
 //   k_solid_to_mushy   ConvertSolid1(1) (S/VoxelsCA.C:780-797)
 //   k_active_flags     active-set membership (S/VoxelsCA.C:262-273) + per-block counts
-//   k_scan             per-instance exclusive scan of the block counts
+//   k_scan_chunks,     two-level exclusive scan of the block counts
+//   k_scan
 //   k_compact          ordered active list, voxel->active map, gathers, ruq draws
-//                      (S/VoxelsCA.C:274-303, 336-380)
-//   k_active_setup     active-neighbour table (S/VoxelsCA.C:304-335), growing list and
-//                      capture candidates (the per-pair part of S/VoxelsCA.C:386-445)
+//                      (S/VoxelsCA.C:274-303, 336-380); k_loop_rng seeds the loop's engine
+//   k_active_setup     active-neighbour table (S/VoxelsCA.C:304-335) and growing list;
+//   k_grow_lists       the growing cells' sorted candidate lists (the per-pair part of
+//                      S/VoxelsCA.C:386-445) with each entry's capture re-centring
 //   k_event_loop       the serial capture/nucleation loop (S/VoxelsCA.C:381-610), one
 //                      workgroup per instance, FP64 throughout
 //   k_scatter          active arrays back to the grid (S/VoxelsCA.C:633-643)
 //   k_mushy_to_solid   ConvertSolid1(0) over the active cells (S/VoxelsCA.C:739-758)
+// Per-voxel passes take runs of 16 voxels per thread (one 16-byte state load).  Layer
+// kernels: k_voronoi, k_site_rank, k_write_layer (base plate and powder layers,
+// S/BasePlate.C, S/VoxelsCA.C AddLayer1/CleanLayer); k_grain_presence, k_remap (grain
+// renumbering).
 // Grid-wide kernels take a flattened block list over all instances, so a batch of
 // domains of different sizes is one launch per stage.
 //
