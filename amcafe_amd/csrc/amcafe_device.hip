@@ -1606,6 +1606,12 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         hc[2 * HN + s] = h.r1;
       }
       if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
+      // bring the rest of the list (positions 16..25 of the distances and ruqs, the
+      // targets' line) into the cache: later head refreshes read it within the event
+      const size_t cb = (size_t)s * AMCD_NB;
+      const double w0 = gmem(d.cD)[cb + 16], w1 = gmem(d.cR)[cb + 16];
+      const int w2 = gmem(d.cTgt)[cb + 25];
+      asm volatile("" ::"v"(w0), "v"(w1), "v"(w2));
     }
   }
   int skip = -1;  // voxel the engine may be capturing right now
