@@ -132,6 +132,8 @@ struct Inst {
   float2* gLH;      // per slot: capture-clock bounds (lo, hi) of the memory-resident tier
   int2* gHd;        // per slot: list head (entry, target) of the memory-resident tier
   uint8_t *gN, *gDead, *gGrow;
+  unsigned* gdead;  // liveness bitmap of the capture loop in memory (one bit per active cell),
+                    // for interfaces larger than the LDS bitmap holds
   double *cD, *cF, *cR;
   double4* cRC;     // per entry of a set-up list: the capture's re-centred centroid and extent
   int *cTgt, *cV;   // per entry: target (active index) and its voxel
@@ -1994,7 +1996,9 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
   evals_out += evals;
 }
 
-template <int NT, int K>
+// GD: the liveness bitmap lives in memory (d.gdead) instead of LDS, for interfaces of more
+// than AMCD_DEAD_BITS cells (instantiated with the memory-resident tier only).
+template <int NT, int K, bool GD = false>
 __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, amcd_out* O) {
   constexpr int NW = NT / 64;
   // slots this tier holds: K per owner thread in registers, or (K = 0) every slot of the
@@ -2002,7 +2006,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const int CAP = K > 0 ? (NT - 64) * K : I[blockIdx.x].acap;
   __shared__ EngineShared sh;
   __shared__ int s_go;
-  __shared__ unsigned s_dead[AMCD_DEAD_BITS / 32];
+  __shared__ unsigned s_dead[GD ? 1 : AMCD_DEAD_BITS / 32];
   __shared__ double s_hc[HeadCache<K>::on ? 3 * (NT - 64) * K : 1];
   // The descriptor and step record are copied to LDS: every event ends in a barrier, after
   // which the compiler must re-read anything in global memory, and each array access would
@@ -2021,14 +2025,15 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const amcd_params& p = d.p;
   const amcd_step& st = s_step;
   const int Na = ls->Na;
-  if (Na <= AMCD_DEAD_BITS) {  // liveness bitmap from the current states
+  unsigned* const dead = GD ? d.gdead : s_dead;
+  if (GD || Na <= AMCD_DEAD_BITS) {  // liveness bitmap from the current states
     for (int wd = tid; wd < (Na + 31) / 32; wd += NT) {
       unsigned m = 0;
       for (int q = 0; q < 32; ++q) {
         const int a = 32 * wd + q;
         if (a >= Na || d.aVS[a] != 1) m |= 1u << q;
       }
-      s_dead[wd] = m;
+      dead[wd] = m;
     }
   }
   if (tid == 0) {
@@ -2040,8 +2045,8 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     sh.fin = 0;
     sh.r_i1 = -1;
     s_go = 1;
-    if (Na > AMCD_DEAD_BITS) {
-      s_go = -3;  // more interface cells than the liveness bitmap holds
+    if (!GD && Na > AMCD_DEAD_BITS) {
+      s_go = -3;  // more interface cells than the LDS bitmap holds: the host relaunches with GD
     } else if (ls->nslots > CAP) {
       s_go = -2;  // more growing cells than this tier holds
     } else if (ls->pending) {
@@ -2056,9 +2061,9 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   long long visits = 0, evals = 0;
   if (w > 0) {
     if constexpr (K > 0)
-      owner_loop<NT, K>(d, sh, s_dead, s_hc, go, ls->tinc, visits, evals);
+      owner_loop<NT, K>(d, sh, dead, s_hc, go, ls->tinc, visits, evals);
     else
-      owner_loop_mem<NT>(d, sh, s_dead, go, ls->tinc, visits, evals);
+      owner_loop_mem<NT>(d, sh, dead, go, ls->tinc, visits, evals);
   } else {
     // engine state, identical in every lane of wave 0
     long long e_visits = 0, e_evals = 0, e_units = 0;
@@ -2202,13 +2207,13 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           d.aExt[i1] = E0;
           d.aVS[i1] = 2;
           d.aScan[i1] = scan;
-          s_dead[i1 >> 5] |= 1u << (i1 & 31);
+          dead[i1 >> 5] |= 1u << (i1 & 31);
         }
         vSind2++;
         tinc += x.dt;
         PHASE(et, 3, tid == 0);
         // the new mushy voxel joins the growing list if it has liquid neighbours
-        const bool liq = live_target(s_dead, tn, i1);
+        const bool liq = live_target(dead, tn, i1);
         const unsigned long long bal = __ballot(liq);
         if (cold && bal) {
           const int slot = nslots;
@@ -2550,7 +2555,7 @@ struct amcd_batch {
   bool timing = false;
   int tier = 1;  // capture-loop capacity tier (see launch_event_loop)
   int min_tier = 0;  // lowest tier a step starts in (AMCAFE_CAPTURE_TIER_MIN; tests use it to
-                     // run small decks in the larger tiers)
+                     // run small decks in the larger tiers; 5 = memory tier, bitmap in memory)
 };
 
 static thread_local std::string g_dev_err;
@@ -2584,7 +2589,7 @@ static int alloc_active(amcd_batch* b, Inst& d, int acap) {
       dalloc(b, &d.cTgt, c * AMCD_NB) || dalloc(b, &d.cK, c * AMCD_NB) || dalloc(b, &d.aRN, c * AMCD_NB) ||
       dalloc(b, &d.gR, c * 9) || dalloc(b, &d.gC, c * 3) || dalloc(b, &d.gV, c) || dalloc(b, &d.gG, c) ||
       dalloc(b, &d.cV, c * AMCD_NB) || dalloc(b, &d.gLH, c) || dalloc(b, &d.gHd, c) ||
-      dalloc(b, &d.cRC, c * AMCD_NB))
+      dalloc(b, &d.cRC, c * AMCD_NB) || dalloc(b, &d.gdead, (c + 31) / 32))
     return -1;
   return 0;
 }
@@ -2618,7 +2623,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   b->n = n;
   b->h.resize(n);
   if (const char* mt = getenv("AMCAFE_CAPTURE_TIER_MIN")) {
-    b->min_tier = std::max(0, std::min(4, atoi(mt)));
+    b->min_tier = std::max(0, std::min(5, atoi(mt)));  // 5: the memory tier with the bitmap in memory
     b->tier = std::max(b->tier, b->min_tier);
   }
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -2958,8 +2963,10 @@ static int launch_update(amcd_batch* b, bool fused_temperature) {
 }
 
 // Capacity tiers of the capture loop: threads per workgroup x growing cells per thread.
+static int launch_event_loop_wide(amcd_batch* b);
 static int launch_event_loop(amcd_batch* b) {
   const int n = b->n;
+  if (b->tier >= 5) return launch_event_loop_wide(b);
   switch (b->tier) {
     case 0:
       hipLaunchKernelGGL((k_event_loop<256, 4>), dim3(n), dim3(256), 0, b->stream, b->d_inst, b->d_steps, b->d_outs);
@@ -2980,6 +2987,13 @@ static int launch_event_loop(amcd_batch* b) {
                          b->d_outs);
       break;
   }
+  HIPT(hipGetLastError());
+  return 0;
+}
+// the memory-resident tier with the liveness bitmap in memory (interfaces beyond AMCD_DEAD_BITS)
+static int launch_event_loop_wide(amcd_batch* b) {
+  hipLaunchKernelGGL((k_event_loop<512, 0, true>), dim3(b->n), dim3(512), 0, b->stream, b->d_inst, b->d_steps,
+                     b->d_outs);
   HIPT(hipGetLastError());
   return 0;
 }
@@ -3070,6 +3084,17 @@ static int grow_active(amcd_batch* b, const amcd_out* outs) {
   return 0;
 }
 
+// An instance whose interface is larger than the LDS liveness bitmap returns from the
+// launch before its loop starts (status -3); the loop then runs for it in the memory-
+// resident tier with the bitmap in memory (instances already done return at once).
+static int widen(amcd_batch* b, amcd_out* outs) {
+  bool need = false;
+  for (int i = 0; i < b->n; ++i) need = need || outs[i].status == -3;
+  if (!need) return 0;
+  if (launch_event_loop_wide(b) || read_outs(b, outs)) return -1;
+  return 0;
+}
+
 // Relaunch (resume) the capture loop with a larger tier while some instance has more
 // growing cells than the current tier holds.
 static int regrow_tier(amcd_batch* b, amcd_out* outs) {
@@ -3114,7 +3139,7 @@ static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, 
     if (!ovf) break;
     if (grow_active(b, outs)) return -1;
   }
-  if (regrow_tier(b, outs)) return -1;
+  if (widen(b, outs) || regrow_tier(b, outs)) return -1;
   bool pend = false;
   for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
   if (!pend) {
@@ -3138,7 +3163,7 @@ int amcd_resume(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
   if (put_steps(b, steps)) return -1;
   if (launch_event_loop(b)) return -1;
   if (read_outs(b, outs)) return -1;
-  if (regrow_tier(b, outs)) return -1;
+  if (widen(b, outs) || regrow_tier(b, outs)) return -1;
   bool pend = false;
   for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
   if (!pend) {

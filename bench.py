@@ -42,6 +42,8 @@ WORKLOADS = {
     "c1_raster": "512x512x128 voxels, dx 2.5 um, 4 layers of 30 um, 3-track bidirectional raster, "
                  "hatch 60 um, 90 deg rotation, 250 W, 1.0 m/s, FP64",
     "c0_single_track": "128x64x32 voxels, dx 2.5 um, one layer, one track at 0.8 m/s, FP64",
+    "c4_production": "1024x512x128 voxels, dx 2.5 um, 8 layers of 30 um over a 5000-grain base plate, "
+                     "22-track bidirectional raster, hatch 60 um, 67 deg rotation, 250 W, 1.0 m/s, FP64",
 }
 
 

@@ -85,11 +85,12 @@ def test_odd_sized_deck_full_run():
 
 
 @pytest.mark.parametrize("deck,tier", [("t_small.in", 2), ("t_small.in", 3), ("t_small.in", 4),
-                                       ("c0_single_track.in", 4)])
+                                       ("c0_single_track.in", 4), ("t_small.in", 5), ("c0_single_track.in", 5)])
 def test_capture_tiers_full_run(monkeypatch, deck, tier):
     """Small decks run with the capture loop held in a larger tier (AMCAFE_CAPTURE_TIER_MIN):
     2 = eight cells per owner thread with the LDS head cache, 3 = sixteen cells per thread
-    reading heads from the lists, 4 = every cell's state in memory.  Every step against the
+    reading heads from the lists, 4 = every cell's state in memory, 5 = the same with the
+    liveness bitmap in memory (interfaces beyond the LDS bitmap).  Every step against the
     oracle."""
     monkeypatch.setenv("AMCAFE_CAPTURE_TIER_MIN", str(tier))
     n = _lockstep(deck, 10**6)
@@ -146,6 +147,32 @@ def test_seam_level_api_per_step():
         if o.finish_step():
             break
     assert events > 100
+
+
+def test_c4_production_lockstep_from_mid_run_state():
+    """The production run of the calibration-and-production form at full size (1024x512x128,
+    5000-grain base plate, 67 deg layer rotation): the GPU's state after 120 steps, taken over
+    by the correctly rounded oracle; the next two steps must agree field for field."""
+    deck = read("c4_production.in")
+    sim = amcafe_amd.Sim(deck)
+    for _ in range(120):
+        sim.step()
+    f = sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID"))
+    o = Oracle(deck, crmath=True, bare=True)
+    o.load_state(f, sim.grains(0), sim.info(0))
+    for n in range(2):
+        sim.step()
+        o.temperature()
+        o.update_voxels()
+        o.finish_step()
+        g, of = sim.fields(0), o.fields()
+        for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
+            assert _same(g[k], of[k]), f"c4 step {121 + n}: " + _diff(k, g[k], of[k])
+        s, so = sim.stats(0), o.stats()
+        print(f"c4 step {121 + n}: {s['n_events']} events, {s['n_active']} interface cells")
+        assert s["n_events"] == so["n_events"] > 0
+        assert s["n_active"] == so["n_active"]
+    assert np.array_equal(sim.grains(0), o.grains())
 
 
 def test_c1_full_size_lockstep_from_mid_run_state():
