@@ -6,4 +6,4 @@ with ctypes; it has no compute path of its own and raises if the library is abse
 """
 from ._lib import lib, LibraryMissing, AmcafeError  # noqa: F401
 from .sim import Sim, Domain, read_deck  # noqa: F401
-from .heat import HeatField, track_desc  # noqa: F401
+from .heat import HeatBatch, HeatField, calibrate, track_desc  # noqa: F401

@@ -104,11 +104,12 @@ int amcd_init_plate(amcd_batch* b, int i, int nzh, const double* xs, int ng);
 // iz1..ilaserLoc-1 takes grains gbase + 1..; *n_used = sites that label a voxel
 int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, const double* xs, int ng, int* n_used);
 
-// differentiable temperature field (heat_adjoint.hip): mode 0 loss (FP32), 1 loss and
-// gradient (FP32), 2 loss (FP64); amcafe_heat_desc is declared in include/amcafe.h
+// differentiable temperature field (heat_adjoint.hip) for nb instances on one grid and track
+// (theta, target, grad: 4 per instance; loss: one per instance): mode 0 loss (FP32), 1 loss
+// and gradient (FP32), 2 loss (FP64); amcafe_heat_desc is declared in include/amcafe.h
 struct amcafe_heat_desc_s;
-int amcd_heat(const struct amcafe_heat_desc_s* d, const double* theta, const double* target, int step0, int nsteps,
-              int mode, double* loss, double* grad, float* ms, char* err, int errlen);
+int amcd_heat(const struct amcafe_heat_desc_s* d, int nb, const double* theta, const double* target, int step0,
+              int nsteps, int mode, double* loss, double* grad, float* ms, char* err, int errlen);
 // EASM temperature (S/TempField.C:79-137) instead of the analytic one: beam power Q,
 // layer thickness; outs[i].any_melt = 1 when some voxel is below tL; *n_evals (may be
 // NULL) = source-integral evaluations summed over the batch

@@ -28,6 +28,8 @@
 #include <math.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/amcafe.h"
 #include "amcafe_dev.h"
 #include "gl25.h"
@@ -97,10 +99,14 @@ __device__ inline F block_sum(F v, F* sh) {
   return t;
 }
 
-// one thread per voxel, steps step0+1 .. step0+nsteps; out[0] += loss, out[1..4] += grad
+// one thread per voxel and instance (blockIdx.y), steps step0+1 .. step0+nsteps; for
+// instance b, out[8b] += loss, out[8b+1..8b+4] += grad
 template <typename F, bool GRAD>
-__global__ __launch_bounds__(256) void k_heat(Geo<F> g, Src<F> s, Src<F> tgt, int step0, int nsteps, F* out) {
+__global__ __launch_bounds__(256) void k_heat(Geo<F> g, const Src<F>* S, const Src<F>* Tg, int step0, int nsteps,
+                                              F* out) {
   __shared__ F sh[4];
+  const Src<F> s = S[blockIdx.y], tgt = Tg[blockIdx.y];
+  out += 8 * blockIdx.y;
   const int v = blockIdx.x * 256 + threadIdx.x;
   const int n = g.nx * g.ny * g.nz;
   F loss = 0, gr[4] = {0, 0, 0, 0};
@@ -168,46 +174,58 @@ Src<F> src(const double* th) {
     }                                                                      \
   } while (0)
 
-// segments of at most `seg` steps, one launch each; sums accumulate on the device in F
+// segments of at most `seg` steps, one launch each (all instances of the batch in it); sums
+// accumulate on the device in F
 template <typename F, bool GRAD>
-int run(const amcafe_heat_desc* d, const double* theta, const double* target, int step0, int nsteps, double* loss,
-        double* grad, float* ms, char* err, int errlen) {
+int run(const amcafe_heat_desc* d, int nb, const double* theta, const double* target, int step0, int nsteps,
+        double* loss, double* grad, float* ms, char* err, int errlen) {
   const int n = d->nX[0] * d->nX[1] * d->nX[2];
   const int seg = d->checkpoint > 0 ? d->checkpoint : 100;
   F* out = nullptr;
-  HCHECK(hipMalloc(&out, 8 * sizeof(F)));
-  HCHECK(hipMemset(out, 0, 8 * sizeof(F)));
+  Src<F>* dsrc = nullptr;
+  std::vector<Src<F>> hs(2 * (size_t)nb);
+  for (int b = 0; b < nb; ++b) {
+    hs[b] = src<F>(theta + 4 * b);
+    hs[nb + b] = src<F>(target + 4 * b);
+  }
+  HCHECK(hipMalloc(&out, 8 * sizeof(F) * nb));
+  HCHECK(hipMalloc(&dsrc, sizeof(Src<F>) * 2 * nb));
+  HCHECK(hipMemcpy(dsrc, hs.data(), sizeof(Src<F>) * 2 * nb, hipMemcpyHostToDevice));
+  HCHECK(hipMemset(out, 0, 8 * sizeof(F) * nb));
   hipEvent_t e0, e1;
   HCHECK(hipEventCreate(&e0));
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipEventRecord(e0, 0));
   const Geo<F> g = geo<F>(d);
-  const Src<F> s = src<F>(theta), t = src<F>(target);
   for (int m = 0; m < nsteps; m += seg) {
     const int c = nsteps - m < seg ? nsteps - m : seg;
-    hipLaunchKernelGGL((k_heat<F, GRAD>), dim3((n + 255) / 256), dim3(256), 0, 0, g, s, t, step0 + m, c, out);
+    hipLaunchKernelGGL((k_heat<F, GRAD>), dim3((n + 255) / 256, nb), dim3(256), 0, 0, g, dsrc, dsrc + nb,
+                       step0 + m, c, out);
     HCHECK(hipGetLastError());
   }
   HCHECK(hipEventRecord(e1, 0));
-  F h[8];
-  HCHECK(hipMemcpy(h, out, sizeof(h), hipMemcpyDeviceToHost));
+  std::vector<F> h(8 * (size_t)nb);
+  HCHECK(hipMemcpy(h.data(), out, sizeof(F) * 8 * nb, hipMemcpyDeviceToHost));
   float el = 0;
   HCHECK(hipEventElapsedTime(&el, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   HCHECK(hipFree(out));
-  if (loss) *loss = (double)h[0];
-  if (GRAD && grad)
-    for (int q = 0; q < 4; ++q) grad[q] = (double)h[1 + q];
+  HCHECK(hipFree(dsrc));
+  for (int b = 0; b < nb; ++b) {
+    if (loss) loss[b] = (double)h[8 * b];
+    if (GRAD && grad)
+      for (int q = 0; q < 4; ++q) grad[4 * b + q] = (double)h[8 * b + 1 + q];
+  }
   if (ms) *ms = el;
   return 0;
 }
 
 }  // namespace
 
-extern "C" int amcd_heat(const amcafe_heat_desc* d, const double* theta, const double* target, int step0, int nsteps,
-                         int mode, double* loss, double* grad, float* ms, char* err, int errlen) {
-  if (mode == 0) return run<float, false>(d, theta, target, step0, nsteps, loss, nullptr, ms, err, errlen);
-  if (mode == 1) return run<float, true>(d, theta, target, step0, nsteps, loss, grad, ms, err, errlen);
-  return run<double, false>(d, theta, target, step0, nsteps, loss, nullptr, ms, err, errlen);
+extern "C" int amcd_heat(const amcafe_heat_desc* d, int nb, const double* theta, const double* target, int step0,
+                         int nsteps, int mode, double* loss, double* grad, float* ms, char* err, int errlen) {
+  if (mode == 0) return run<float, false>(d, nb, theta, target, step0, nsteps, loss, nullptr, ms, err, errlen);
+  if (mode == 1) return run<float, true>(d, nb, theta, target, step0, nsteps, loss, grad, ms, err, errlen);
+  return run<double, false>(d, nb, theta, target, step0, nsteps, loss, nullptr, ms, err, errlen);
 }

@@ -78,6 +78,61 @@ class HeatField:
         return g
 
 
+class HeatBatch:
+    """Many parameter sets on one grid and track, each against its own target: loss and
+    adjoint gradient of all of them in one launch per checkpoint segment."""
+
+    def __init__(self, desc, targets):
+        self.desc = desc
+        self.targets = [list(t) for t in targets]
+
+    def loss_grad_f32(self, thetas, step0, nsteps):
+        n = len(self.targets)
+        if len(thetas) != n:
+            raise ValueError("one parameter set per target")
+        th = (ctypes.c_double * (4 * n))(*[float(v) for t in thetas for v in t])
+        tg = (ctypes.c_double * (4 * n))(*[float(v) for t in self.targets for v in t])
+        loss, g, ms = (ctypes.c_double * n)(), (ctypes.c_double * (4 * n))(), ctypes.c_float()
+        check(lib().amcafe_heat_loss_grad_batch_f32(ctypes.byref(self.desc), n, th, tg, step0, nsteps, loss, g,
+                                                    ctypes.byref(ms)))
+        return list(loss), [list(g[4 * b:4 * b + 4]) for b in range(n)], ms.value
+
+
+def calibrate(desc, targets, theta0, iters=40, lr=0.08, step0=0, nsteps=None, on_iter=None):
+    """Fit one parameter set (P, v, eta, sigma) per target field by gradient descent with the
+    adjoint gradient: Adam on the logarithms of the parameters (all positive), every instance
+    advanced in the same launches.  P and eta enter the field only as the product eta*P, so the
+    fit determines eta*P, v and sigma; the split of eta*P stays where theta0 puts it.
+    Returns (thetas, losses per iteration)."""
+    import math
+
+    n = len(targets)
+    nsteps = nsteps if nsteps is not None else 0
+    batch = HeatBatch(desc, targets)
+    z = [[math.log(v) for v in theta0] for _ in range(n)]
+    m = [[0.0] * 4 for _ in range(n)]
+    s2 = [[0.0] * 4 for _ in range(n)]
+    b1, b2, eps = 0.9, 0.999, 1e-12
+    history = []
+    for it in range(1, iters + 1):
+        thetas = [[math.exp(v) for v in zb] for zb in z]
+        loss, grad, _ = batch.loss_grad_f32(thetas, step0, nsteps)
+        history.append(loss)
+        if on_iter:
+            on_iter(it, thetas, loss)
+        for b in range(n):
+            for q in range(4):
+                gq = grad[b][q] * thetas[b][q]  # d loss / d log theta
+                m[b][q] = b1 * m[b][q] + (1 - b1) * gq
+                s2[b][q] = b2 * s2[b][q] + (1 - b2) * gq * gq
+                mh, vh = m[b][q] / (1 - b1 ** it), s2[b][q] / (1 - b2 ** it)
+                z[b][q] -= lr * mh / (math.sqrt(vh) + eps)
+    thetas = [[math.exp(v) for v in zb] for zb in z]
+    loss, _, _ = batch.loss_grad_f32(thetas, step0, nsteps)
+    history.append(loss)
+    return thetas, history
+
+
 def heat_loss_function():
     """torch.autograd.Function over HeatField (imported lazily: torch is optional)."""
     import torch

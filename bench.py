@@ -124,6 +124,34 @@ def adjoint_cost(fd_steps=2000):
             "grad": grad, "fd_grad": fd, "fd_steps": fd_steps, "max_rel_err_vs_fd": max(rel)}
 
 
+def calibration_run(n=32, iters=60, steps=400):
+    """The calibration form (opt-in): n instances on 128x128x48 voxels (dx 2.5 um), one +x track
+    of `steps` steps, each fitted by Adam on the log-parameters with the adjoint gradient of
+    all instances in the same launches, to its own target drawn uniformly (seed 7) from power
+    150-300 W, speed 0.5-1.5 m/s, absorptivity 0.3-0.5, radius 40-60 um, from (200 W, 0.8 m/s,
+    0.35, 50 um).  eta*P, v and sigma are identifiable (P and eta enter as their product)."""
+    import amcafe_amd
+    rng = np.random.default_rng(7)
+    targets = [[rng.uniform(150, 300), rng.uniform(0.5, 1.5), rng.uniform(0.3, 0.5), rng.uniform(40e-6, 60e-6)]
+               for _ in range(n)]
+    desc = amcafe_amd.track_desc((128, 128, 48), steps=steps, v_ref=1.0, checkpoint=100)
+    t0 = time.perf_counter()
+    thetas, hist = amcafe_amd.calibrate(desc, targets, [200.0, 0.8, 0.35, 50e-6], iters=iters, lr=0.08,
+                                        nsteps=steps)
+    wall = time.perf_counter() - t0
+    red = [hist[-1][b] / hist[0][b] for b in range(n)]
+    err = lambda f: [abs(f(thetas[b]) / f(targets[b]) - 1) for b in range(n)]
+    e_pe, e_v, e_s = err(lambda t: t[0] * t[2]), err(lambda t: t[1]), err(lambda t: t[3])
+    return {"workload": "c4_calibration", "instances": n, "voxels_per_instance": 128 * 128 * 48, "steps": steps,
+            "iterations": iters, "dtype": "f32", "wall_s": round(wall, 2),
+            "ms_per_iteration": round(1e3 * wall / (iters + 1), 1),
+            "loss_ratio_median": float(np.median(red)), "loss_ratio_max": float(max(red)),
+            "rel_err_etaP_median": float(np.median(e_pe)), "rel_err_etaP_max": float(max(e_pe)),
+            "rel_err_v_median": float(np.median(e_v)), "rel_err_v_max": float(max(e_v)),
+            "rel_err_sigma_median": float(np.median(e_s)), "rel_err_sigma_max": float(max(e_s)),
+            "within_5pct": int(sum(max(a, b, c) < 0.05 for a, b, c in zip(e_pe, e_v, e_s)))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +162,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-single comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
+    ap.add_argument("--calibration", action="store_true",
+                    help="also run the calibration form (32 instances, 128x128x48, about a minute)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (scale for HBM counters)")
     args = ap.parse_args()
@@ -220,6 +250,7 @@ def main():
     if not args.no_batched:
         batched = batched_vs_single(os.path.join(ROOT, "decks", "c2_batched"))
     adjoint = None if args.no_adjoint else adjoint_cost()
+    calibration = calibration_run() if args.calibration else None
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
@@ -233,6 +264,8 @@ def main():
                    "growing_cells_per_step": round(tot["n_growing"] / steps, 1)},
         "roofline": roof, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
     }
+    if calibration is not None:
+        out["calibration"] = calibration
     print(json.dumps(out), flush=True)
 
 

@@ -190,6 +190,12 @@ int amcafe_heat_loss_grad_f32(const amcafe_heat_desc* d, const double* theta, co
 /* FP64 forward (finite differences) */
 int amcafe_heat_loss_f64(const amcafe_heat_desc* d, const double* theta, const double* target, int step0,
                          int nsteps, double* loss, float* ms);
+/* n instances on the same grid and track in one launch per checkpoint segment (the calibration
+ * form): theta, target and grad hold 4 values per instance, loss one (1 <= n <= 65535).  Each
+ * instance's result is that of amcafe_heat_loss_grad_f32 on its own parameters, up to the order
+ * of the FP32 block sums. */
+int amcafe_heat_loss_grad_batch_f32(const amcafe_heat_desc* d, int n, const double* theta, const double* target,
+                                    int step0, int nsteps, double* loss, double* grad, float* ms);
 
 #ifdef __cplusplus
 }
