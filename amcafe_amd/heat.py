@@ -98,18 +98,12 @@ class HeatBatch:
         return list(loss), [list(g[4 * b:4 * b + 4]) for b in range(n)], ms.value
 
 
-def calibrate(desc, targets, theta0, iters=40, lr=0.08, step0=0, nsteps=None, on_iter=None):
-    """Fit one parameter set (P, v, eta, sigma) per target field by gradient descent with the
-    adjoint gradient: Adam on the logarithms of the parameters (all positive), every instance
-    advanced in the same launches.  P and eta enter the field only as the product eta*P, so the
-    fit determines eta*P, v and sigma; the split of eta*P stays where theta0 puts it.
-    Returns (thetas, losses per iteration)."""
+def _adam(batch, thetas0, iters, lr, step0, nsteps):
+    """Adam on the log-parameters of every instance of the batch; (thetas, losses per iteration)."""
     import math
 
-    n = len(targets)
-    nsteps = nsteps if nsteps is not None else 0
-    batch = HeatBatch(desc, targets)
-    z = [[math.log(v) for v in theta0] for _ in range(n)]
+    n = len(thetas0)
+    z = [[math.log(v) for v in t] for t in thetas0]
     m = [[0.0] * 4 for _ in range(n)]
     s2 = [[0.0] * 4 for _ in range(n)]
     b1, b2, eps = 0.9, 0.999, 1e-12
@@ -118,8 +112,6 @@ def calibrate(desc, targets, theta0, iters=40, lr=0.08, step0=0, nsteps=None, on
         thetas = [[math.exp(v) for v in zb] for zb in z]
         loss, grad, _ = batch.loss_grad_f32(thetas, step0, nsteps)
         history.append(loss)
-        if on_iter:
-            on_iter(it, thetas, loss)
         for b in range(n):
             for q in range(4):
                 gq = grad[b][q] * thetas[b][q]  # d loss / d log theta
@@ -130,6 +122,40 @@ def calibrate(desc, targets, theta0, iters=40, lr=0.08, step0=0, nsteps=None, on
     thetas = [[math.exp(v) for v in zb] for zb in z]
     loss, _, _ = batch.loss_grad_f32(thetas, step0, nsteps)
     history.append(loss)
+    return thetas, history
+
+
+# alternative starts for instances that stop in a local minimum: speed and radius, which shape
+# the melt pool most, spread over the calibration ranges (power and absorptivity kept)
+RESTART_V = (0.6, 1.0, 1.4)
+RESTART_SIGMA = (42e-6, 58e-6)
+
+
+def calibrate(desc, targets, theta0, iters=40, lr=0.08, step0=0, nsteps=None, restart_below=1e-2):
+    """Fit one parameter set (P, v, eta, sigma) per target field by gradient descent with the
+    adjoint gradient: Adam on the logarithms of the parameters (all positive), every instance
+    advanced in the same launches.  P and eta enter the field only as the product eta*P, so the
+    fit determines eta*P, v and sigma; the split of eta*P stays where theta0 puts it.
+    Instances whose loss has not fallen below restart_below times its first value are run again
+    from the starts RESTART_V x RESTART_SIGMA (all of them in one batch) and keep the best fit.
+    Returns (thetas, losses per iteration of the first run)."""
+    n = len(targets)
+    nsteps = nsteps if nsteps is not None else 0
+    thetas, history = _adam(HeatBatch(desc, targets), [list(theta0)] * n, iters, lr, step0, nsteps)
+    best = list(history[-1])
+    stuck = [b for b in range(n) if not (history[-1][b] < restart_below * history[0][b])]
+    if stuck and restart_below > 0:
+        starts = [[theta0[0], v, theta0[2], sg] for v in RESTART_V for sg in RESTART_SIGMA]
+        tg = [targets[b] for b in stuck for _ in starts]
+        th0 = [st for _ in stuck for st in starts]
+        th, hist = _adam(HeatBatch(desc, tg), th0, iters, lr, step0, nsteps)
+        for i, b in enumerate(stuck):
+            for j in range(len(starts)):
+                k = i * len(starts) + j
+                if hist[-1][k] < best[b]:
+                    best[b] = hist[-1][k]
+                    thetas[b] = th[k]
+    history.append(best)
     return thetas, history
 
 

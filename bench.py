@@ -149,7 +149,8 @@ def calibration_run(n=32, iters=60, steps=400):
             "rel_err_etaP_median": float(np.median(e_pe)), "rel_err_etaP_max": float(max(e_pe)),
             "rel_err_v_median": float(np.median(e_v)), "rel_err_v_max": float(max(e_v)),
             "rel_err_sigma_median": float(np.median(e_s)), "rel_err_sigma_max": float(max(e_s)),
-            "within_5pct": int(sum(max(a, b, c) < 0.05 for a, b, c in zip(e_pe, e_v, e_s)))}
+            "within_5pct": int(sum(max(a, b, c) < 0.05 for a, b, c in zip(e_pe, e_v, e_s))),
+            "restarted": int(sum(hist[-2][b] >= 1e-2 * hist[0][b] for b in range(n)))}
 
 
 def main():
