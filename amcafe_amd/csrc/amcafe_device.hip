@@ -1311,49 +1311,98 @@ struct SlotHead {
   double D1, Dn, r1;  // head distance, the next entry's distance (+inf: none), head ruqs
   int c1, tgt1, k1;   // head entry, its target (-1: no live entry), its neighbour rank
 };
-// First live entry at or after position `from` of slot s.  The entries are read four at
-// a time together with the list length, so the common case is one round of loads
-// (positions past the length hold stale data and are masked afterwards).
-__device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead, int s, int from, int skip) {
+// Positions p .. p+3 of a list and the distance at p+4.  Positions past the end of
+// the 26 entries are read at the last entry (always in the slot), so that the loads
+// carry no branches; they lie beyond the list length and are never used.
+struct SlotBlock {
+  double D[5], R[4];
+  int t[4], k[4], n;
+};
+__device__ static inline void block_entries(const Inst& d, size_t cb, int p, SlotBlock& b) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const size_t e = cb + (p + q < AMCD_NB ? p + q : AMCD_NB - 1);
+    b.D[q] = gmem(d.cD)[e];
+    if (q < 4) {
+      b.t[q] = gmem(d.cTgt)[e];
+      b.k[q] = gmem(d.cK)[e];
+      b.R[q] = gmem(d.cR)[e];
+    }
+  }
+}
+// The first live entry among positions p .. p+3 of a loaded block into h: true when
+// found or when the list ends before p (h unchanged), false to go on at p + 4.
+__device__ static inline bool block_pick(const SlotBlock& b, const unsigned* dead, size_t cb, int p, int skip,
+                                         SlotHead& h) {
+  if (p >= b.n) return true;
+  int hit = -1;
+#pragma unroll
+  for (int q = 3; q >= 0; --q)
+    if (p + q < b.n && live_target(dead, b.t[q], skip)) hit = q;
+  if (hit < 0) return false;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (q == hit) {
+      h.c1 = (int)cb + p + q;
+      h.tgt1 = b.t[q];
+      h.D1 = b.D[q];
+      h.Dn = (p + q + 1 < b.n) ? b.D[q + 1] : __builtin_inf();
+      h.r1 = b.R[q];
+      h.k1 = b.k[q];
+    }
+  return true;
+}
+__device__ static inline SlotHead head_none(size_t cb, int n) {
   SlotHead h;
-  const int n = gmem(d.gN)[s];
-  const size_t cb = (size_t)s * AMCD_NB;
   h.c1 = (int)cb + n;
   h.tgt1 = -1;
   h.k1 = 0;
   h.D1 = h.Dn = h.r1 = __builtin_inf();
+  return h;
+}
+// First live entry at or after position `from` of slot s.  The entries are read four at
+// a time together with the list length, so the common case is one round of loads
+// (positions past the length hold stale data and are masked afterwards).
+__device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead, int s, int from, int skip) {
+  const size_t cb = (size_t)s * AMCD_NB;
+  SlotBlock b;
+  b.n = gmem(d.gN)[s];
+  SlotHead h = head_none(cb, b.n);
   for (int p = from; p < AMCD_NB; p += 4) {
-    int t[4], kq[4];
-    double D[5], R[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const bool in = p + q < AMCD_NB;
-      t[q] = in ? gmem(d.cTgt)[cb + p + q] : -1;
-      D[q] = in ? gmem(d.cD)[cb + p + q] : 0.0;
-      R[q] = in ? gmem(d.cR)[cb + p + q] : 0.0;
-      kq[q] = in ? gmem(d.cK)[cb + p + q] : 0;
-    }
-    D[4] = (p + 4 < AMCD_NB) ? gmem(d.cD)[cb + p + 4] : 0.0;
-    if (p >= n) break;
-    int hit = -1;
-#pragma unroll
-    for (int q = 3; q >= 0; --q)
-      if (p + q < n && live_target(dead, t[q], skip)) hit = q;
-    if (hit >= 0) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (q == hit) {
-          h.c1 = (int)cb + p + q;
-          h.tgt1 = t[q];
-          h.D1 = D[q];
-          h.Dn = (p + q + 1 < n) ? D[q + 1] : __builtin_inf();
-          h.r1 = R[q];
-          h.k1 = kq[q];
-        }
-      break;
-    }
+    block_entries(d, cb, p, b);
+    if (block_pick(b, dead, cb, p, skip, h)) break;
   }
   return h;
+}
+// slot_head's first block (positions p .. p+3, the distance at p+4 and the list length)
+// loaded ahead of its use, and the same search started from it.
+__device__ static inline SlotBlock slot_block(const Inst& d, int s, int p) {
+  SlotBlock b;
+  b.n = gmem(d.gN)[s];
+  block_entries(d, (size_t)s * AMCD_NB, p, b);
+  return b;
+}
+__device__ static inline SlotBlock slot_block_none() {
+  SlotBlock b;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    b.t[q] = -1;
+    b.k[q] = 0;
+    b.D[q] = b.R[q] = 0.0;
+  }
+  b.D[4] = 0.0;
+  b.n = 0;
+  return b;
+}
+// slot_head(d, dead, s, from, skip) with its first block b = slot_block(d, s, from)
+// already loaded: the same result, the later blocks read only when the first holds no
+// live entry
+__device__ static inline SlotHead slot_head_pre(const Inst& d, const unsigned* dead, int s, int from, int skip,
+                                                const SlotBlock& b) {
+  const size_t cb = (size_t)s * AMCD_NB;
+  SlotHead h = head_none(cb, b.n);
+  if (from >= AMCD_NB || block_pick(b, dead, cb, from, skip, h)) return h;
+  return slot_head(d, dead, s, from + 4, skip);
 }
 
 // Float bounds [lo, hi] on tau = T + t, the event clock at which the cell's head would
@@ -1626,6 +1675,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   x.i1 = sh.r_i1;
   x.dt = 0.0;
   bool apply_first = go == 2;
+  int ks = -1;  // the cell whose list block pb is loaded for its head refresh (-1: none)
+  SlotBlock pb = slot_block_none();
   PHASE_INIT(ot);
   while (go > 0) {
     // The slot numbers are re-derived from an opaque copy of the thread index in every
@@ -1702,12 +1753,27 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       __syncthreads();
       PHASE(ot, 8, tid == 64);
       // ---- DECIDE
-      x = decide<NW>(sh, par, T, p.DelT, p.rX, nd);
+      const Best eb = lane_entry<NW>(sh, par);
+      x = decide_from<NW>(sh, eb, par, T, p.DelT, p.rX, nd);
       if (x.stop) break;
       ++nd;
       PHASE(ot, 5, tid == 64);
     }
     apply_first = false;
+    // the cell (first of each thread) whose head is the voxel captured now: the next
+    // block of its list is loaded before the growth and the takeover below, and waited
+    // for where the refresh of its head uses it
+    if constexpr (HC) {
+      ks = -1;
+      int pk = 0;
+#pragma unroll
+      for (int k = K - 1; k >= 0; --k) {
+        const bool take = (tgt1[k] == x.i1) & (tgt1[k] >= 0);
+        ks = take ? k : ks;
+        pk = take ? c1[k] : pk;
+      }
+      if (ks >= 0) pb = slot_block(d, mo + ks * NO, (pk & 31) + 1);
+    }
     // ---- APPLY
     const double Dt = x.dt;
     if (!x.nuc) {  // growth of every cell growing at this scan (S/VoxelsCA.C:511-516)
@@ -1746,6 +1812,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         hc[HN + nc.slot] = nc.Dn1;
         hc[2 * HN + nc.slot] = nc.r1;
       }
+      if (ks == kk) ks = -1;  // a block loaded for the slot's previous state
     }
     // heads captured now, clamped cells, the new cell, and every cell once per 2^20 events
     ++nev;
@@ -1762,7 +1829,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         const int s = mo + k * NO;
         if (tgt1[k] == i1) {
           const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
-          const SlotHead h = slot_head(d, dead, s, pos + 1, i1);
+          const SlotHead h = (HC && k == ks) ? slot_head_pre(d, dead, s, pos + 1, i1, pb)
+                                             : slot_head(d, dead, s, pos + 1, i1);
           tgt1[k] = h.tgt1;
           if constexpr (HC) {
             c1[k] = (h.c1 - s * AMCD_NB) | ((((c1[k] >> 5) & ~31) + h.k1) << 5);
