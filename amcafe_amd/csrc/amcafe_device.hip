@@ -1311,22 +1311,23 @@ struct SlotHead {
   double D1, Dn, r1;  // head distance, the next entry's distance (+inf: none), head ruqs
   int c1, tgt1, k1;   // head entry, its target (-1: no live entry), its neighbour rank
 };
-// Positions p .. p+3 of a list and the distance at p+4.  Positions past the end of
-// the 26 entries are read at the last entry (always in the slot), so that the loads
-// carry no branches; they lie beyond the list length and are never used.
+// Positions p .. p+3 of a list and the distance at p+4 (p <= 26), read from one base
+// address per array with no branches.  Positions past a slot's 26 entries read the next
+// slot, or for the last slot the 256 bytes every device array is allocated beyond its
+// length; they lie beyond the list length and are never used.
 struct SlotBlock {
   double D[5], R[4];
   int t[4], k[4], n;
 };
 __device__ static inline void block_entries(const Inst& d, size_t cb, int p, SlotBlock& b) {
+  const size_t e = cb + p;
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
-    const size_t e = cb + (p + q < AMCD_NB ? p + q : AMCD_NB - 1);
-    b.D[q] = gmem(d.cD)[e];
+    b.D[q] = gmem(d.cD)[e + q];
     if (q < 4) {
-      b.t[q] = gmem(d.cTgt)[e];
-      b.k[q] = gmem(d.cK)[e];
-      b.R[q] = gmem(d.cR)[e];
+      b.t[q] = gmem(d.cTgt)[e + q];
+      b.k[q] = gmem(d.cK)[e + q];
+      b.R[q] = gmem(d.cR)[e + q];
     }
   }
 }
@@ -1513,23 +1514,19 @@ __device__ static inline Decision decide_from(const EngineShared& sh, Best e, in
   b.cand = __builtin_amdgcn_readlane(e.cand, 0);
   b.tgt = __builtin_amdgcn_readlane(e.tgt, 0);
   const int wq = __builtin_amdgcn_readlane(wi, 0);
+  // branch-free, so that the reads of the uniform and the suspension flag are issued
+  // with the read of the bests and not one after another
   Decision x;
-  x.stop = 0;
-  x.nuc = 0;
+  const bool run = T < DelT;                      // loop condition (S/VoxelsCA.C:381)
+  const bool none = (b.cand < 0) | (b.t >= 1e6);  // S/VoxelsCA.C:480
+  const bool go = run & (susp == 0) & !none;
+  const bool nuc = U < rX;
+  x.stop = !run ? 1 : (susp != 0 ? susp : (none ? 1 : 0));
+  x.nuc = go & nuc;
   x.i1 = b.tgt;
   x.cand = b.cand;
   x.wave = wq;
-  x.dt = 0.0;
-  if (!(T < DelT)) {
-    x.stop = 1;  // loop condition (S/VoxelsCA.C:381)
-  } else if (susp) {
-    x.stop = susp;
-  } else if (b.cand < 0 || b.t >= 1e6) {
-    x.stop = 1;  // S/VoxelsCA.C:480
-  } else {
-    x.nuc = U < rX;
-    x.dt = x.nuc ? 0.0 : b.t;
-  }
+  x.dt = (go & !nuc) ? b.t : 0.0;
   return x;
 }
 template <int NW>
@@ -1630,6 +1627,9 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   const int me = tid - 64;
   const float fInf = __builtin_inff();
   const amcd_params& p = d.p;
+  // loop constants held in registers (a read through the descriptor in LDS would be
+  // repeated after every barrier)
+  const double DelT = p.DelT, rX = p.rX;
   double E[K], V[K];
   float lo[K], hi[K];
   // c1: the head entry; with the head cache, its position in the slot's list (5 bits)
@@ -1754,7 +1754,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       PHASE(ot, 8, tid == 64);
       // ---- DECIDE
       const Best eb = lane_entry<NW>(sh, par);
-      x = decide_from<NW>(sh, eb, par, T, p.DelT, p.rX, nd);
+      x = decide_from<NW>(sh, eb, par, T, DelT, rX, nd);
       if (x.stop) break;
       ++nd;
       PHASE(ot, 5, tid == 64);
@@ -1880,6 +1880,9 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
   const int me = tid - 64;
   const float fInf = __builtin_inff();
   const amcd_params& p = d.p;
+  // loop constants held in registers (a read through the descriptor in LDS would be
+  // repeated after every barrier)
+  const double DelT = p.DelT, rX = p.rX;
   int ns = go != -2 ? sh.nslots : 0;  // slots this lane knows of (new ones arrive by mailbox)
   // The lane's smallest upper bound and its two smallest lower bounds (with their slots)
   // and third smallest lower bound, gathered whenever every cell's bounds pass through the
@@ -1988,7 +1991,7 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
       b = wave_best(b);
       if (lane == 0) sh.sb[par][w] = b;
       __syncthreads();
-      x = decide<NW>(sh, par, T, p.DelT, p.rX, nd);
+      x = decide<NW>(sh, par, T, DelT, rX, nd);
       if (x.stop) break;
       ++nd;
     }
@@ -2091,6 +2094,9 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   LoopState* ls = d.ls;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const amcd_params& p = d.p;
+  // loop constants held in registers (a read through the descriptor in LDS would be
+  // repeated after every barrier)
+  const double DelT = p.DelT, rX = p.rX;
   const amcd_step& st = s_step;
   const int Na = ls->Na;
   unsigned* const dead = GD ? d.gdead : s_dead;
@@ -2170,7 +2176,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         {
           const Best me = lane_entry<NW>(sh, par);
           pf = engine_prefetch<NW>(d, sh, me, par, ns_pre);
-          x = decide_from<NW>(sh, me, par, tinc, p.DelT, p.rX, nd);
+          x = decide_from<NW>(sh, me, par, tinc, DelT, rX, nd);
           prefetch_landed(pf);
         }
         have_pf = true;
@@ -2365,7 +2371,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             nc.valid = slot < CAP;  // beyond this tier: resume with a larger one
           }
           nslots = slot + 1;
-          if (slot >= CAP && tinc < p.DelT) susp = -2;
+          if (slot >= CAP && tinc < DelT) susp = -2;
         }
         PHASE(et, 4, tid == 0);
       }
@@ -2636,6 +2642,8 @@ static thread_local std::string g_dev_err;
     }                                                                                  \
   } while (0)
 
+// Every device array has 256 bytes beyond its length: the list-block reads of the
+// capture loop run up to four entries past the last slot's list.
 template <class T>
 static int dalloc(amcd_batch* b, T** p, size_t n) {
   void* q = nullptr;
