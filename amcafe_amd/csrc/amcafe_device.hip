@@ -1315,18 +1315,22 @@ struct SlotHead {
 // address per array with no branches.  Positions past a slot's 26 entries read the next
 // slot, or for the last slot the 256 bytes every device array is allocated beyond its
 // length; they lie beyond the list length and are never used.
+// The four neighbour ranks stay packed in one word until a head is picked: unpacked at
+// the load, they would make the loads of a block that is issued early wait there.
 struct SlotBlock {
   double D[5], R[4];
-  int t[4], k[4], n;
+  int t[4], n;
+  unsigned kp;  // rank of position p + q in bits 8q .. 8q+7
 };
+typedef unsigned u32_any_align __attribute__((aligned(1)));
 __device__ static inline void block_entries(const Inst& d, size_t cb, int p, SlotBlock& b) {
   const size_t e = cb + p;
+  b.kp = *gmem((u32_any_align*)(d.cK + e));  // four ranks, little-endian: one load
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     b.D[q] = gmem(d.cD)[e + q];
     if (q < 4) {
       b.t[q] = gmem(d.cTgt)[e + q];
-      b.k[q] = gmem(d.cK)[e + q];
       b.R[q] = gmem(d.cR)[e + q];
     }
   }
@@ -1349,7 +1353,7 @@ __device__ static inline bool block_pick(const SlotBlock& b, const unsigned* dea
       h.D1 = b.D[q];
       h.Dn = (p + q + 1 < b.n) ? b.D[q + 1] : __builtin_inf();
       h.r1 = b.R[q];
-      h.k1 = b.k[q];
+      h.k1 = (int)((b.kp >> (8 * q)) & 0xffu);
     }
   return true;
 }
@@ -1388,11 +1392,11 @@ __device__ static inline SlotBlock slot_block_none() {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     b.t[q] = -1;
-    b.k[q] = 0;
     b.D[q] = b.R[q] = 0.0;
   }
   b.D[4] = 0.0;
   b.n = 0;
+  b.kp = 0;
   return b;
 }
 // slot_head(d, dead, s, from, skip) with its first block b = slot_block(d, s, from)
