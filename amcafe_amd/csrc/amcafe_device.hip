@@ -1372,11 +1372,14 @@ __device__ static inline SlotHead slot_head(const Inst& d, const unsigned* dead,
   const size_t cb = (size_t)s * AMCD_NB;
   SlotBlock b;
   b.n = gmem(d.gN)[s];
-  SlotHead h = head_none(cb, b.n);
+  // the default head is set after the search, so that the length is not waited for
+  // before the entries are requested
+  SlotHead h = head_none(cb, 0);
   for (int p = from; p < AMCD_NB; p += 4) {
     block_entries(d, cb, p, b);
     if (block_pick(b, dead, cb, p, skip, h)) break;
   }
+  if (h.tgt1 < 0) h.c1 = (int)cb + b.n;
   return h;
 }
 // slot_head's first block (positions p .. p+3, the distance at p+4 and the list length)
