@@ -1723,8 +1723,12 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
           }
           visits++;
           evals++;
+          // the head's time and the next entry's, side by side (one round of head reads,
+          // two independent divisions)
           const double t1 = (D1 - e) / v;
-          if (t1 <= 1e6) {
+          const double tn = (Dn - e) / v;
+          const bool ok = t1 <= 1e6;
+          {
             Best y;
             y.t = t1;
             y.r = rr;
@@ -1733,7 +1737,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
             y.tgt = tgt1[k];
             // times are non-decreasing along the list: a tie needs the next entry's
             // time to equal t1; then walk the tied block
-            if ((Dn - e) / v == t1) {
+            if (ok & (tn == t1)) {
               const int n = gmem(d.gN)[s];
               for (int q = pos + 1; q < n; ++q) {
                 const int tq = gmem(d.cTgt)[c - pos + q];
@@ -1750,7 +1754,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
                 if (better(z, y)) y = z;
               }
             }
-            if (better(y, b)) b = y;
+            if (ok && better(y, b)) b = y;
           }
         }
       }
