@@ -2320,12 +2320,14 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             r = rpair;
             key = scan * 32 + lane;
           }
-          int rank = 0;  // position in (D, ruqs, key) order
-#pragma unroll
-          for (int j = 0; j < AMCD_NB; ++j) {
+          // position in (D, ruqs, key) order among the liquid neighbours: a uniform loop
+          // over the ballot's set bits, branch-free inside
+          int rank = 0;
+          for (unsigned long long m = bal; m != 0; m &= m - 1) {
+            const int j = (int)__builtin_ctzll(m);
             const double Dj = lane_f64(D, j), rj = lane_f64(r, j);
             const int kj = __builtin_amdgcn_readlane(key, j);
-            if (((bal >> j) & 1) && j != lane && cand_less(Dj, rj, kj, D, r, key)) rank++;
+            rank += ((j != lane) & cand_less(Dj, rj, kj, D, r, key)) ? 1 : 0;
           }
           if (liq) {
             d.cD[cb + rank] = D;
