@@ -2487,6 +2487,13 @@ __global__ __launch_bounds__(GB) void k_mushy_to_solid(Inst* I, const int* astar
 }
 
 // ---- per-layer helpers ---------------------------------------------------------------
+// After a host upload of vState: the "T >= tL" flag (bit 7) follows the device's current
+// temperature field, so SetLiquid3 still sees the caller's melt pool.
+__global__ void k_rehot(uint8_t* vs, const double* T, int N, double tL) {
+  size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)N) return;
+  vs[v] = (uint8_t)((vs[v] & 0x7f) | (T[v] >= tL ? 0x80 : 0));
+}
 __global__ void k_grain_presence(const int* gid, int N, int* present, int ngrain) {
   size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= (size_t)N) return;
@@ -2782,6 +2789,8 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
       return -1;
     }
     int acap = (int)std::min<long long>(N, std::max<long long>(1 << 16, N / 8));
+    if (const char* ac = getenv("AMCAFE_ACTIVE_CAP_INIT"))  // tests: start small to exercise the growth path
+      acap = (int)std::min<long long>(N, std::max(64, atoi(ac)));
     if (alloc_active(b, d, acap)) {
       snprintf(err, errlen, "%s", g_dev_err.c_str());
       amcd_destroy(b);
@@ -2859,6 +2868,9 @@ int amcd_upload(amcd_batch* b, int i, const int* gid, const int* vstate, const d
     HIPT(hipMemcpy(d.vs, tmp.data(), N, hipMemcpyHostToDevice));
     HIPT(hipMemset(d.zliq, 0xff, (size_t)d.p.nz * 4));
     HIPT(hipMemset(&d.ls->tvalid, 0, sizeof(int)));
+    hipLaunchKernelGGL(k_rehot, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, b->stream, d.vs, d.T, d.N, d.p.tL);
+    HIPT(hipGetLastError());
+    HIPT(hipStreamSynchronize(b->stream));
   }
   if (bj) {
     for (size_t v = 0; v < N; ++v) tmp[v] = (uint8_t)(bj[v] ? 1 : 0);
@@ -3220,7 +3232,10 @@ static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, 
       // fresh output records; the temperature kernel sets any_melt
       HIPT(hipMemsetAsync(b->d_outs, 0, sizeof(amcd_out) * b->n, b->stream));
     }
-    if (launch_update(b, fused || attempt > 0)) return -1;
+    // A retry repeats the step's stages with the same temperature source: the fused form
+    // re-evaluates the same laser (idempotent), the seam form keeps the caller's field and
+    // re-applies SetLiquid3 to it (idempotent for the same T).
+    if (launch_update(b, fused)) return -1;
     if (launch_active(b)) return -1;
     if (read_outs(b, outs)) return -1;
     bool ovf = false;

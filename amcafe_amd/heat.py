@@ -35,6 +35,7 @@ def track_desc(nX, dX=2.5e-6, steps=2000, v_ref=0.8, checkpoint=100, t_hist=8e-4
     d.T0, d.tL, d.width = 300.0, 1733.0, 10.0
     d.k, d.alpha = 20.0, 5e-6
     d.checkpoint = checkpoint
+    d.steps = steps  # the track's length in steps (the default of calibrate's nsteps)
     return d
 
 
@@ -138,9 +139,17 @@ def calibrate(desc, targets, theta0, iters=40, lr=0.08, step0=0, nsteps=None, re
     fit determines eta*P, v and sigma; the split of eta*P stays where theta0 puts it.
     Instances whose loss has not fallen below restart_below times its first value are run again
     from the starts RESTART_V x RESTART_SIGMA (all of them in one batch) and keep the best fit.
+    nsteps defaults to the track's length recorded by track_desc; a missing or non-positive step
+    count is an error (a fit over zero steps has a zero gradient and would return theta0).
     Returns (thetas, losses per iteration of the first run)."""
     n = len(targets)
-    nsteps = nsteps if nsteps is not None else 0
+    if nsteps is None:
+        nsteps = getattr(desc, "steps", None)
+    if nsteps is None or int(nsteps) <= 0:
+        raise ValueError("calibrate needs a positive step count: pass nsteps or build desc with track_desc")
+    nsteps = int(nsteps)
+    if n == 0:
+        raise ValueError("calibrate needs at least one target")
     thetas, history = _adam(HeatBatch(desc, targets), [list(theta0)] * n, iters, lr, step0, nsteps)
     best = list(history[-1])
     stuck = [b for b in range(n) if not (history[-1][b] < restart_below * history[0][b])]

@@ -9,9 +9,14 @@ timed steps, with the domain resident in HBM before the timed region starts.
 Default workload: c1_raster (512x512x128 voxels, 4 layers of 30 um, 3-track
 bidirectional raster, 90 deg layer rotation), the largest single-GPU configuration.
 
-The timed steps run as a caller runs them, with no events between the stages.  The next
-as many steps run with HIP events between the stages and give the per-stage breakdown
-and the dominant kernel's launch time.
+The timed window starts at a fixed point of the run, --start-step (default: step 150 of
+the workload, where the laser is on its first raster track inside the domain and the melt
+pool has its steady size), not at the laser's entry: the run is advanced to that step
+(untimed, reported as config.start_step), then --warmup untimed steps, then --steps timed
+steps.  The timed steps run as a caller runs them, with no events between the stages.  The
+next as many steps run with HIP events between the stages and give the per-stage breakdown
+and the dominant kernel's launch time.  full_run is the whole workload run from its first
+step to its last in a fresh simulation (layer changes included), with its own wall time.
 
 Also reported, in the same JSON line:
   roofline      the dominant kernel's algorithmic bytes per launch / its mean launch
@@ -19,6 +24,7 @@ Also reported, in the same JSON line:
                 against 8 TB/s HBM3E
   cpu_baseline  the oracle (a one-rank CPU port of the reference path) timed on the
                 host for a few steps starting from the GPU's state before the timed region
+                (one thread; the reference's N-rank MPI figure is not measured)
   batched       the 64 decks of decks/c2_batched (64^3 .. 256x128x96 voxels, one track,
                 1-2 layers) run to completion as one batch and then one by one: aggregate
                 cell-updates/s of each and the ratio
@@ -158,14 +164,33 @@ def calibration_run(n=32, iters=60, steps=400):
             "restarted": int(sum(hist[-2][b] >= 1e-2 * hist[0][b] for b in range(n)))}
 
 
+def full_run(deck):
+    """The whole workload from its first step to its last in a fresh Sim (layer changes, host
+    layer routines and output bookkeeping included), set-up excluded."""
+    import amcafe_amd
+    sim = amcafe_amd.Sim(deck)
+    info = sim.info()
+    nvox = info["nX"][0] * info["nX"][1] * info["nX"][2]
+    t0 = time.perf_counter()
+    steps = sim.run()
+    wall = time.perf_counter() - t0
+    layers = sim.info()["indlayer"] - 1
+    sim.close()
+    return {"steps": steps, "layers": layers, "wall_s": round(wall, 3), "ms_per_step": round(1e3 * wall / steps, 4),
+            "value": round(nvox * steps / wall, 1), "unit": "cell-updates/s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=150)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--start-step", type=int, default=150,
+                    help="step the run is advanced to (untimed) before the warm-up and timed steps")
     ap.add_argument("--workload", default="c1_raster", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-full-run", action="store_true", help="skip the whole-run measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-single comparison")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation-form measurement")
     ap.add_argument("--calibration", action="store_true",
@@ -195,8 +220,13 @@ def main():
     t_setup = time.perf_counter() - t_setup
     info = sim.info()
     nvox = info["nX"][0] * info["nX"][1] * info["nX"][2]
+    t_ff = time.perf_counter()
+    for _ in range(args.start_step):
+        sim.step()
+    t_ff = time.perf_counter() - t_ff
     for _ in range(args.warmup):
         sim.step()
+    first_timed = sim.info()["tInd"]
     snap = None
     if not args.no_cpu:
         snap = (sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID")), sim.grains(0),
@@ -236,15 +266,15 @@ def main():
     if abytes is not None:
         per_launch = abytes / tsteps
         achieved = per_launch / (launch_ms * 1e-3) / 1e9
-        traffic = None
-        tfile = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{args.workload}.json")
-        if os.path.exists(tfile):
-            with open(tfile) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+        # HBM bytes per launch are measured only by rocprofv3 counter runs, which cannot run
+        # inside this process; the summaries of such runs are kept beside the kernel traces
         roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
                 "kernel": KERNEL_NAMES[dom], "launch_ms": round(launch_ms, 4),
-                "algorithmic_bytes_per_launch": round(per_launch, 1)}
+                "algorithmic_bytes_per_launch": round(per_launch, 1),
+                "window": f"steps {first_timed + steps}-{first_timed + steps + tsteps - 1} (HIP events)"}
+    ev = tot["n_events"] / tsteps
+    cap_us_per_event = (stage_ms[4] / tsteps) * 1e3 / ev if ev > 0 else None
 
     cpu = None
     if snap is not None:
@@ -260,9 +290,12 @@ def main():
             orc.finish_step()
         tc = time.perf_counter() - tc
         cpu = {"value": round(nvox * args.cpu_steps / tc, 1), "unit": "cell-updates/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of {args.workload} from the state after {args.warmup} warmup "
-                         f"steps, one thread, {tc:.1f} s"}
+               "nproc": os.cpu_count(),
+               "sample": f"{args.cpu_steps} steps of {args.workload} from step {first_timed} (the GPU's state at "
+                         f"the start of the timed window), one thread, {tc:.1f} s; the reference's N-rank MPI "
+                         f"figure is not measured"}
 
+    whole = None if args.no_full_run else full_run(deck)
     batched = None
     if not args.no_batched:
         batched = batched_vs_single(os.path.join(ROOT, "decks", "c2_batched"))
@@ -275,12 +308,15 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": args.workload, "description": WORKLOADS[args.workload], "voxels": nvox,
                    "instances": 1, "setup_s": round(t_setup, 2),
+                   "start_step": args.start_step, "fast_forward_s": round(t_ff, 3),
+                   "timed_steps": f"{first_timed}-{first_timed + steps - 1}",
                    "stage_ms_per_step": dict(zip(STAGES, np.round(stage_ms / tsteps, 4).tolist())),
                    "stage_steps": tsteps,
-                   "events_per_step": round(tot["n_events"] / tsteps, 1),
+                   "events_per_step": round(ev, 1),
+                   "capture_us_per_event": None if cap_us_per_event is None else round(cap_us_per_event, 3),
                    "active_cells_per_step": round(tot["n_active"] / tsteps, 1),
                    "growing_cells_per_step": round(tot["n_growing"] / tsteps, 1)},
-        "roofline": roof, "cpu_baseline": cpu, "batched": batched, "adjoint": adjoint,
+        "roofline": roof, "cpu_baseline": cpu, "full_run": whole, "batched": batched, "adjoint": adjoint,
     }
     if calibration is not None:
         out["calibration"] = calibration

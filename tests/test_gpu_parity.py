@@ -116,7 +116,14 @@ def test_seam_level_api_per_step():
     """The seam entry points (amcafe_domain_*: upload, AnalyticTempCurr, UpdateVoxels,
     download, grains) against the oracle, one step at a time from the oracle's state,
     over the whole small deck including its layer changes."""
-    deck = read("t_small.in")
+    assert seam_run("t_small.in", 400) > 100
+
+
+def seam_run(deck_name, max_steps, upload_after_temperature=False):
+    """Drive the seam level from the oracle's state every step; returns the events seen.
+    With upload_after_temperature the state is uploaded between the temperature and the
+    UpdateVoxels calls (SetLiquid3 must still see the melt pool of the device's field)."""
+    deck = read(deck_name)
     kv = _deck_values(deck)
     nX = [int(v) for v in kv["nx"]]
     tL = float(kv["tl"][0])
@@ -125,14 +132,17 @@ def test_seam_level_api_per_step():
                             float(kv["r"][0]), float(kv["dendritegrowth2"][0]), float(kv["dendritegrowth2"][1]))
     o = Oracle(deck, crmath=True)
     events = 0
-    for step in range(400):
+    for step in range(max_steps):
         f = o.fields()
         s = o.scalars()
-        dom.upload(gID=f["gID"], vState=f["vState"], extents=f["extents"], centroidOct=f["centroidOct"],
-                   BuildjID=f["BuildjID"])
-        dom.set_grains(o.grains())
         x, y, z, dirp, gth, il = o.laser_inputs()
-        any_melt = dom.temperature(x, y, z, dirp, gth, il)
+        if upload_after_temperature:
+            any_melt = dom.temperature(x, y, z, dirp, gth, il)
+        dom.upload(gID=f["gID"], vState=f["vState"], extents=f["extents"], centroidOct=f["centroidOct"],
+                   BuildjID=None if upload_after_temperature else f["BuildjID"])
+        dom.set_grains(o.grains())
+        if not upload_after_temperature:
+            any_melt = dom.temperature(x, y, z, dirp, gth, il)
         o.temperature()
         st = dom.update_voxels(int(s["tInd"]), il)
         o.update_voxels()
@@ -146,7 +156,7 @@ def test_seam_level_api_per_step():
         events += st["n_events"]
         if o.finish_step():
             break
-    assert events > 100
+    return events
 
 
 def test_c4_production_lockstep_from_mid_run_state():
