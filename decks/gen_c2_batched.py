@@ -8,7 +8,10 @@ Mapping onto the reference deck grammar (S/Grid.C:195-317), per instance:
   nX ~ U{64..256}, nY ~ U{64..128}, nZ ~ U{64..96}; dx 2.5e-6
   layers L ~ U{1,2} of 30 um (lT 30e-6, 12 voxels); bpH = (nZ - 12 L) dx so that
       nlayerTot = ceil((nZ - Nzh)/12) = L (S/main1.C:59)
-  one straight track along x through the domain centre: patternID 1, gridsize nX*dx 0
+  one straight track along x through the domain centre: patternID 1, gridsize (nX*dx + 2*(bhatch/2
+      + tail + 20 um)) 0: the track overhangs the domain by more than bhatch/2 plus the melt
+      pool's tail, so its melt pool clears before the pattern ends and a second layer really
+      starts one layer higher (S/Grid.C:137-151) instead of rescanning the first
   bmP ~ U[150,300] W (0.1 W), bmV ~ U[0.5,1.5] m/s (1 mm/s); bmDelT = dx/bmV (one voxel per step);
       tS = tL - (0.5 bmV / Avel)^(1/3) (max growth per step 0.5 dx at the maximum undercooling)
   nucleation density n: log-uniform on [1e13,1e15] m^-3 (uniform in the exponent), r = n*1e-18 um^-3
@@ -43,6 +46,7 @@ def deck(i, nx, ny, nz, layers, P, v, n_m3):
     if mp is None:
         raise RuntimeError(f"instance {i}: no melt pool at P={P} v={v}")
     front, tail, hw, depth = (round(q * 1e6) for q in mp)
+    gx = round(nx * 2.5 + 2 * (30 + tail + 20), 1) * 1e-6
     return f"""# Batched workload "c2" instance {i}: {nx}x{ny}x{nz} voxels, {layers} layer(s), P {P} W, v {v} m/s,
 # nucleation density {n_m3:.4g} m^-3 (written by decks/gen_c2_batched.py)
 dx 2.5e-6
@@ -63,7 +67,7 @@ patternID 1
 outint 100000
 outNL 1
 offset 0.0 0.0 0.0
-gridsize {nx * DX!r} 0
+gridsize {gx!r} 0
 gridtheta 0
 dendritegrowth2 2e-5 3
 """

@@ -402,6 +402,10 @@ struct Sim {
     snaps.push_back(Snapshot{TtInd, gID, vState, Build, TempCurr, cTheta});
   }
   StepStats st;
+  // sites of the last Voronoi tessellation (base plate or powder layer) and, for the last
+  // AddLayer1: iz1, ilaserLoc, sites drawn, labels used (i2), orientation seed, nGrain before
+  std::vector<double> last_sites;
+  long last_layer[6] = {0, 0, 0, 0, 0, 0};
   // Moore offsets in ascending linear-index order (S/Grid.C:399-414 + sort at :375)
   int noff = 0;
 
@@ -479,6 +483,7 @@ struct Sim {
       Xs[3 * j + 1] = xrand(bp_generator) * Ly;
       Xs[3 * j + 2] = xrand(bp_generator) * bp_height;
     }
+    last_sites = Xs;
     int nplate = Nzh * g.nX[0] * g.nX[1];
     std::vector<int> X(nplate, 0);
     for (int j = 0; j < nplate; ++j) {
@@ -727,15 +732,13 @@ struct Sim {
     xp[2] = x0[2] + (x1[2] - x0[2]) * t;
   }
 
-  // S/VoxelsCA.C:232-655 at one rank
-  void update_voxels() {
-    StepStats s;
-    set_liquid3();
-    convert_solid1(1);
+  // Active set (S/VoxelsCA.C:262-275): sorted unique {mushy, or liquid below tL} plus their
+  // neighbours at or above tL; active-space CSR (S/VoxelsCA.C:304-335): each active cell's
+  // active neighbours as active indices, neighbours in ascending id order.
+  void active_set(std::vector<int>& indA, std::vector<int>& nptr, std::vector<int>& nidx) const {
     const double tL = g.tL;
-    // active set (S/VoxelsCA.C:262-275)
-    std::vector<int> indA;
     int nb[26], cnt;
+    indA.clear();
     for (int j = 0; j < N; ++j) {
       if (vState[j] == 2 || (vState[j] == 1 && TempCurr[j] < tL)) {
         indA.push_back(j);
@@ -747,6 +750,29 @@ struct Sim {
     std::sort(indA.begin(), indA.end());
     indA.erase(std::unique(indA.begin(), indA.end()), indA.end());
     const int Na = (int)indA.size();
+    nptr.assign(Na + 1, 0);
+    nidx.clear();
+    nidx.reserve(8 * (size_t)Na);
+    for (int j = 0; j < Na; ++j) {
+      nptr[j] = (int)nidx.size();
+      nbrs(indA[j], nb, cnt);
+      for (int q = 0; q < cnt; ++q) {
+        auto it = std::lower_bound(indA.begin(), indA.end(), nb[q]);
+        if (it != indA.end() && *it == nb[q]) nidx.push_back((int)(it - indA.begin()));
+      }
+    }
+    nptr[Na] = (int)nidx.size();
+  }
+
+  // S/VoxelsCA.C:232-655 at one rank
+  void update_voxels() {
+    StepStats s;
+    set_liquid3();
+    convert_solid1(1);
+    const double tL = g.tL;
+    std::vector<int> indA, nptr, nidx;
+    active_set(indA, nptr, nidx);
+    const int Na = (int)indA.size();
     s.Na = Na;
     std::vector<double> T(Na), ExtA(Na), Cen(3 * (size_t)Na);
     std::vector<int> vS(Na), G(Na), vI(Na);
@@ -755,18 +781,6 @@ struct Sim {
       vS[j] = vState[i1]; vI[j] = i1; G[j] = gID[i1]; T[j] = TempCurr[i1]; ExtA[j] = extents[i1];
       for (int q = 0; q < 3; ++q) Cen[3 * (size_t)j + q] = centroidOct[3 * (size_t)i1 + q];
     }
-    // active-space CSR (S/VoxelsCA.C:304-335): neighbours in ascending id order
-    std::vector<int> nptr(Na + 1, 0), nidx;
-    nidx.reserve(8 * (size_t)Na);
-    for (int j = 0; j < Na; ++j) {
-      nptr[j] = (int)nidx.size();
-      nbrs(indA[j], nb, cnt);
-      for (int q = 0; q < cnt; ++q) {
-        auto it = std::lower_bound(vI.begin(), vI.end(), nb[q]);
-        if (it != vI.end() && *it == nb[q]) nidx.push_back((int)(it - vI.begin()));
-      }
-    }
-    nptr[Na] = (int)nidx.size();
     // random numbers (S/VoxelsCA.C:336-380)
     unsigned int sdloc = seed0 + 32 * g.tInd + 64 * nGrain;
     Engine g1(30 * g.tInd + seed1);
@@ -978,6 +992,8 @@ struct Sim {
       Xs[3 * j + 1] = xrand(genlayer) * Ly;
       Xs[3 * j + 2] = xrand(genlayer) * g.layerT + zloc0;
     }
+    last_sites = Xs;
+    const int nGrain_before = nGrain;
     int i3 = nx * ny;
     std::vector<int> itmp(ng2, 0), gtmp((size_t)i3 * g.nZlayer, 0);
     int i2 = 0;
@@ -1013,6 +1029,8 @@ struct Sim {
       }
     }
     nGrain += i2 + 1;
+    last_layer[0] = iz1; last_layer[1] = g.ilaserLoc; last_layer[2] = ng2; last_layer[3] = i2;
+    last_layer[4] = sdloc; last_layer[5] = nGrain_before;
     std::vector<double> aa;
     sa.generate(i2 + 1, sdloc, aa);
     cTheta.insert(cTheta.end(), aa.begin(), aa.end());
@@ -1213,6 +1231,38 @@ long orc_events(void* h, int* vox, int* src, double* dt, long cap) {
   return n;
 }
 // standalone sampler and engine draws, for known-answer tests
+// the active set and its CSR for the current state (call orc_state_in first to apply
+// SetLiquid3 and ConvertSolid1(1) as UpdateVoxels does); returns Na, sets *nnz.
+// Arrays may be null to query the sizes.
+long orc_active_set(void* h, int* indA, int* nptr, int* nidx, long* nnz) {
+  std::vector<int> a, p, x;
+  ((orc::Sim*)h)->active_set(a, p, x);
+  if (indA) std::copy(a.begin(), a.end(), indA);
+  if (nptr) std::copy(p.begin(), p.end(), nptr);
+  if (nidx) std::copy(x.begin(), x.end(), nidx);
+  if (nnz) *nnz = (long)x.size();
+  return (long)a.size();
+}
+// sites (x, y, z per site) of the last Voronoi tessellation; returns the site count
+long orc_last_sites(void* h, double* xs) {
+  const orc::Sim* s = (const orc::Sim*)h;
+  if (xs) std::copy(s->last_sites.begin(), s->last_sites.end(), xs);
+  return (long)(s->last_sites.size() / 3);
+}
+// the last AddLayer1: iz1, ilaserLoc, sites drawn, labels used, orientation seed, nGrain before
+void orc_last_layer(void* h, long* out6) {
+  const orc::Sim* s = (const orc::Sim*)h;
+  for (int q = 0; q < 6; ++q) out6[q] = s->last_layer[q];
+}
+void orc_set_build(void* h) { ((orc::Sim*)h)->set_build(); }
+void orc_clean_layer(void* h) { ((orc::Sim*)h)->clean_layer(); }
+void orc_convert_solid(void* h, int iswitch) { ((orc::Sim*)h)->convert_solid1(iswitch); }
+void orc_set_liquid(void* h) { ((orc::Sim*)h)->set_liquid3(); }
+void orc_set_build_flags(void* h, const int* build) {
+  orc::Sim* s = (orc::Sim*)h;
+  if (build) std::copy(build, build + s->N, s->Build.begin());
+}
+
 void orc_generate_samples(int n, unsigned int seed, double* out) {
   orc::Orient o;
   std::vector<double> v;

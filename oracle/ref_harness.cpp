@@ -10,8 +10,18 @@
 // Partition.C, main1.C) includes GSL, ADIOS2 or METIS headers that this image
 // does not have, so it is not built.
 //
+// Also built: src/src-mpi/BasePlate.C, which compiles unchanged.  Its constructor takes a
+// Partition and reads only the two members icellidLoc and ncellLoc; Partition's own
+// constructor lives in Partition.C, which needs METIS and is not built.  The harness
+// therefore sets up those two members of a Partition-typed object itself (the one-rank
+// identity partition of S/Partition.C:281-288) and calls no Partition member function.
+//
 // Usage:
 //   ref_harness samples <n> <seed>            -> n lines "omega nx ny nz" (%a)
+//   ref_harness nucs <n> <seed> <stride>      -> n lines: GenerateSamples(1, seed + stride*k)
+//   ref_harness plate <deck>                  -> BasePlate::GenerateVoxel at one rank: Ngrain,
+//                                                then the label of every plate voxel, then
+//                                                the orientations (omega nx ny nz, %a)
 //   ref_harness grid <deck> <flags-file>      -> parsed deck, then one line per step
 // The flags file holds, per step, the reference's end-of-scan input "is any voxel
 // at or above the liquidus" (0/1); the harness applies the end-of-scan rule of
@@ -24,7 +34,11 @@
 #include <vector>
 #include <cmath>
 #include "mpi.h"
+#include <new>
+#include <numeric>
 #include "Grid.h"
+#include "BasePlate.h"
+#include "Partition.h"
 #include "SampleOrientation.h"
 
 static void print_grid(const Grid& g) {
@@ -54,6 +68,40 @@ int main(int argc, char** argv) {
     std::vector<double> out;
     sa.GenerateSamples(n, seed, out);
     for (int i = 0; i < n; ++i) printf("%a %a %a %a\n", out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+    return 0;
+  }
+  if (mode == "nucs" && argc == 5) {
+    int n = atoi(argv[2]);
+    unsigned int seed = (unsigned int)strtoul(argv[3], nullptr, 10);
+    unsigned int stride = (unsigned int)strtoul(argv[4], nullptr, 10);
+    SampleOrientation sa;
+    std::vector<double> out;
+    for (int k = 0; k < n; ++k) {
+      sa.GenerateSamples(1, seed + stride * (unsigned int)k, out);
+      printf("%a %a %a %a\n", out[0], out[1], out[2], out[3]);
+    }
+    return 0;
+  }
+  if (mode == "plate" && argc == 3) {
+    MPI_Init(NULL, NULL);
+    int myid = 0, np = 1;
+    std::string deck = argv[2];
+    Grid g(deck, myid, np);
+    const int N = g.nX[0] * g.nX[1] * g.nX[2];
+    // zeroed storage: every std::vector member of libstdc++ starts as an empty vector
+    alignas(Partition) static unsigned char pbuf[sizeof(Partition)];
+    Partition* part = reinterpret_cast<Partition*>(pbuf);
+    new (&part->icellidLoc) std::vector<int>(N);
+    std::iota(part->icellidLoc.begin(), part->icellidLoc.end(), 0);
+    part->ncellLoc = N;
+    part->nGhost = 0;
+    BasePlate bp(g, *part);
+    bp.GenerateVoxel();
+    printf("Ngrain %d Nzh %d nlabels %zu\n", bp.Ngrain, bp.Nzh, bp.X.size());
+    for (size_t j = 0; j < bp.X.size(); ++j) printf("%d\n", bp.X[j]);
+    for (int j = 0; j < bp.Ngrain; ++j)
+      printf("%a %a %a %a\n", bp.cTheta[4 * j], bp.cTheta[4 * j + 1], bp.cTheta[4 * j + 2], bp.cTheta[4 * j + 3]);
+    MPI_Finalize();
     return 0;
   }
   if (mode == "grid" && argc == 4) {

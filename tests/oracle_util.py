@@ -50,6 +50,15 @@ def _load(path):
     L.orc_rot_mat.argtypes = [ctypes.c_double, _P, _P]
     L.orc_shape_factor_n.argtypes = [_P, _I, _P]
     L.orc_set_crmath.argtypes = [_I]
+    L.orc_active_set.argtypes = [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_long)]
+    L.orc_active_set.restype = ctypes.c_long
+    L.orc_last_sites.argtypes = [_P, _P]
+    L.orc_last_sites.restype = ctypes.c_long
+    L.orc_last_layer.argtypes = [_P, _P]
+    for f in ("orc_set_build", "orc_clean_layer", "orc_set_liquid"):
+        getattr(L, f).argtypes = [_P]
+    L.orc_convert_solid.argtypes = [_P, _I]
+    L.orc_set_build_flags.argtypes = [_P, _P]
     return L
 
 
@@ -184,6 +193,47 @@ class Oracle:
     def done(self):
         return self.L.orc_done(self.h)
 
+    # ---- single routines of the step, for the rule pins ----
+    def state_in(self):
+        """SetLiquid3 then ConvertSolid1(1), as UpdateVoxels starts."""
+        self.L.orc_state_in(self.h)
+
+    def set_liquid(self):
+        self.L.orc_set_liquid(self.h)
+
+    def convert_solid(self, iswitch):
+        self.L.orc_convert_solid(self.h, iswitch)
+
+    def active_set(self):
+        """(indA, nptr, nidx) of the current state: the active voxels in ascending id order and
+        the CSR of each one's active neighbours (as active indices)."""
+        nnz = ctypes.c_long()
+        na = self.L.orc_active_set(self.h, None, None, None, ctypes.byref(nnz))
+        a, p, x = np.zeros(na, np.int32), np.zeros(na + 1, np.int32), np.zeros(max(nnz.value, 1), np.int32)
+        self.L.orc_active_set(self.h, a.ctypes.data, p.ctypes.data, x.ctypes.data, ctypes.byref(nnz))
+        return a, p, x[:nnz.value]
+
+    def last_sites(self):
+        n = self.L.orc_last_sites(self.h, None)
+        xs = np.zeros(3 * n)
+        self.L.orc_last_sites(self.h, xs.ctypes.data)
+        return xs.reshape(n, 3)
+
+    def last_layer(self):
+        o = np.zeros(6, np.int64)
+        self.L.orc_last_layer(self.h, o.ctypes.data)
+        return dict(zip(("iz1", "ilaserLoc", "ng2", "i2", "sdloc", "nGrain_before"), o.tolist()))
+
+    def set_build(self):
+        self.L.orc_set_build(self.h)
+
+    def set_build_flags(self, build):
+        b = np.ascontiguousarray(build, dtype=np.int32)
+        self.L.orc_set_build_flags(self.h, b.ctypes.data)
+
+    def clean_layer(self):
+        self.L.orc_clean_layer(self.h)
+
     def load_state(self, fields, grains, info):
         """Take over a state produced elsewhere (fields in the reference layouts, the grain
         table and the laser / layer scalars of the driver)."""
@@ -203,3 +253,17 @@ def deck_path(name):
 def read(name):
     with open(deck_path(name)) as f:
         return f.read()
+
+
+def tiny_deck(nX=(6, 5, 4), **over):
+    """A deck text for bare oracles whose state a test sets itself.  Keys as in the
+    reference's deck grammar (S/Grid.C:195-317); `over` replaces or adds keys."""
+    kv = {"dx": "2.5e-6", "nX": " ".join(str(v) for v in nX), "bmV": "0.8", "bmP": "200", "bmDelT": "3.125e-6",
+          "r": "0", "lT": "5e-6", "tL": "1733", "tS": "1705.856", "bhatch": "30e-6", "mu": "1e-3",
+          "bpH": "2.5e-6", "lrate": "1e-3", "meltparam": "20e-6 60e-6 15e-6 12e-6", "patternID": "1",
+          "outint": "100000", "outNL": "1", "offset": "0.0 0.0 0.0",
+          "gridsize": f"{nX[0] * 2.5e-6!r} 0", "gridtheta": "0", "dendritegrowth2": "2e-5 3"}
+    for k, v in over.items():
+        kv[k] = v if isinstance(v, str) else " ".join(repr(float(x)) for x in v) if hasattr(v, "__len__") \
+            else repr(v)
+    return "\n".join(f"{k} {v}" for k, v in kv.items()) + "\n"
