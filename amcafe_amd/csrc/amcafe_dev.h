@@ -61,7 +61,7 @@ typedef struct {
   long long draws;     // engine draws consumed (2 per uniform)
   double tinc;
   int overflow;        // active list exceeded capacity
-  int err_nonanalytic; // a growing cell had a temperature with no host-computed velocity
+  int err_state;        // a growing cell carried no valid grain id (the domain state is corrupt)
   long long slot_visits;  // growing cells with live candidates, summed over the loop's scans
   long long cand_evals;   // capture times evaluated, summed over the loop's scans
   long long interface_events;  // growing-list length summed over the loop's events
@@ -134,6 +134,8 @@ int amcd_uniform_stream(unsigned long long seed, int n, double* out);
 int amcd_uniform_jump(unsigned long long seed, int n, double* out);
 // host build of the device's shape factor pow(cos th sin ph, .5), n vectors xyz
 int amcd_host_shape_factor(const double* dnx, int n, double* out);
+double amcd_host_pow(double x, double y);
+int amcd_host_pow_n(const double* x, const double* y, int n, double* out);
 
 #ifdef __cplusplus
 }

@@ -944,12 +944,13 @@ __device__ static inline void candidate(const Inst& d, const double* c, const do
   *f = ff;
 }
 
-__device__ static inline double growth_velocity(const amcd_params& p, double T, int* err) {
-  if (T >= p.tL) return 0.0;  // S/VoxelsCA.C:405
+// vhat = Avel * pow(tL - T, nvel) (S/VoxelsCA.C:405), pow rounded to nearest; the two
+// temperatures of the analytic model are precomputed, any other (EASM) is evaluated here.
+__device__ static inline double growth_velocity(const amcd_params& p, double T) {
+  if (T >= p.tL) return 0.0;
   if (T == p.tS) return p.vhat_tS;
   if (T == p.T0) return p.vhat_T0;
-  *err = 1;
-  return __builtin_nan("");
+  return p.Avel * amc_pow_cr(p.tL - T, p.nvel);
 }
 
 // Active-neighbour table and, for every growing cell (mushy, some liquid active
@@ -1000,17 +1001,15 @@ __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart,
   if (!grow) return;
   unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int slot = base + __popcll(bal & lt);
-  int err = 0;
   d.gAct[slot] = a;
   d.gScan[slot] = d.aScan[a];
   d.gExt[slot] = d.aExt[a];
-  d.gVhat[slot] = growth_velocity(d.p, d.aT[a], &err);
+  d.gVhat[slot] = growth_velocity(d.p, d.aT[a]);
   d.gDead[slot] = 0;
-  if (err) atomicOr(&O[ii].err_nonanalytic, 1);
   double c[3] = {d.aCx[a], d.aCy[a], d.aCz[a]};
   const int G = d.aG[a];
   if (G < 1 || G > d.gcap) {  // a growing cell must carry a valid grain
-    atomicOr(&O[ii].err_nonanalytic, 2);
+    atomicOr(&O[ii].err_state, 1);
     d.gN[slot] = 0;
     d.gDead[slot] = 1;
     return;
@@ -2303,8 +2302,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         if (cold && bal) {
           const int slot = nslots;
           const size_t cb = (size_t)slot * AMCD_NB;
-          int err = 0;
-          const double V0 = growth_velocity(p, T1, &err);
+          const double V0 = growth_velocity(p, T1);
           double D = __builtin_inf(), f = 0, r = __builtin_inf();
           int key = 0x7fffffff;
           // neighbour lane of v1, in the order of the active-neighbour table
@@ -2377,7 +2375,6 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             d.gVhat[slot] = V0;
             d.gN[slot] = (uint8_t)__popcll(bal);
             d.gDead[slot] = 0;
-            if (err) atomicOr(&O[blockIdx.x].err_nonanalytic, 1);
             nc.E = E0;
             nc.V = V0;
             nc.slot = slot;
@@ -3346,6 +3343,13 @@ int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, cons
   HIPT(hipFree(dx));
   HIPT(hipFree(lab));
   HIPT(hipFree(used));
+  return 0;
+}
+
+// pow(x, y) rounded to nearest, the host build of the device routine (velocity table)
+double amcd_host_pow(double x, double y) { return amc_pow_cr(x, y); }
+int amcd_host_pow_n(const double* x, const double* y, int n, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = amc_pow_cr(x[i], y[i]);
   return 0;
 }
 

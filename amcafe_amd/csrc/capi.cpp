@@ -32,7 +32,7 @@ static const int kSeed0 = 2132512;          // VoxelsCA::seed0 (S/VoxelsCA.C:47)
 static const unsigned int kSeed1 = 2912351; // VoxelsCA::seed1 (S/VoxelsCA.C:48)
 static const int kPoolChunk = 8;            // nucleation orientations prepared per step
 
-// per-domain parameters the kernels read (host libm for the transcendental ones)
+// per-domain parameters the kernels read
 static amcd_params make_params(const int* nX, double dx, double tL, double tS, double T0, const double* a1,
                                double bmV, double DelT, double rNmax, double Avel, double nvel) {
   amcd_params p;
@@ -51,8 +51,10 @@ static amcd_params make_params(const int* nX, double dx, double tL, double tS, d
   p.bmDX0 = DelT * bmV;                     // S/TempField.C:31
   p.DelT = DelT;
   p.rX = rNmax * pow(dx * 1e6, 3.);         // S/VoxelsCA.C:344
-  p.vhat_tS = Avel * pow(tL - tS, nvel);    // S/VoxelsCA.C:405 at T = tS
-  p.vhat_T0 = Avel * pow(tL - T0, nvel);    // ... and at T = T0
+  // S/VoxelsCA.C:405 at T = tS and T = T0, pow rounded to nearest (the device evaluates
+  // the same routine at any other temperature)
+  p.vhat_tS = Avel * amcd_host_pow(tL - tS, nvel);
+  p.vhat_T0 = Avel * amcd_host_pow(tL - T0, nvel);
   p.Avel = Avel;
   p.nvel = nvel;
   p.seed1 = kSeed1;
@@ -124,9 +126,9 @@ static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<a
     if (amcd_resume(dev, st.data(), out.data())) return dev_fail();
   }
   for (int i = 0; i < n; ++i)
-    if (!st[i].skip && out[i].err_nonanalytic)
-      return fail(AMCAFE_ERR_STATE, "a growing cell has a temperature other than tS, T0 or >= tL; the "
-                                    "analytic-model velocity table does not cover it");
+    if (!st[i].skip && out[i].err_state)
+      return fail(AMCAFE_ERR_STATE, "a mushy cell with liquid neighbours carries no valid grain id; the "
+                                    "domain state is corrupt and must be uploaded again");
   return AMCAFE_OK;
 }
 

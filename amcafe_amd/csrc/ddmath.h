@@ -267,3 +267,45 @@ __host__ __device__ static inline __attribute__((always_inline)) double amc_shap
   if (ok) return sqrt(ch * sh);
   return amc_shape_factor_ref(dx, dy, dz);
 }
+
+// ---------------------------------------------------------------------------------
+// pow(x, y) rounded to nearest for x > 0 finite and a result in the normal range: the
+// growth velocity Avel * pow(tL - T, nvel) of S/VoxelsCA.C:405 at any sub-liquidus T.
+// exp(y * log x) in double-double (relative error about 2^-95), then the nearest double;
+// the result is the correctly rounded one unless pow(x, y) lies within that error of a
+// rounding midpoint.
+//   log x = e ln2 + log c + 2 atanh(s),  x = 2^e m, m in [sqrt(1/2), sqrt 2), c = i/64
+//     the table point nearest m, s = (m - c)/(m + c), |s| < 1/180 (terms through s^17);
+//   exp z = 2^k exp(j/64) exp(r),  z = k ln2 + j/64 + r, |r| <= 1/128 (terms through r^12).
+__host__ __device__ static inline amc_dd amc_log_dd(double x) {
+  int e = 0;
+  double m = frexp(x, &e);  // m in [1/2, 1)
+  if (m < 0.70710678118654752) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const int i = (int)rint(m * AMC_DD_TAB);  // 45 .. 91
+  const double c = (double)i / AMC_DD_TAB;
+  const amc_dd num = {m - c, 0.0};           // exact (Sterbenz)
+  const amc_dd den = amc_two_sum(m, c);
+  const amc_dd s = amc_dd_div(num, den);
+  const amc_dd s2 = amc_dd_mul(s, s);
+  amc_dd t = amc_dd_mul(s, amc_poly(kAtanhCoef, 9, s2));
+  t = amc_dd_add(t, t);                       // 2 atanh(s)
+  const amc_dd el = amc_dd_mul_d(amc_tab(kLn2, 0), (double)e);
+  return amc_dd_add(amc_dd_add(el, amc_tab(kLogTab, i - 45)), t);
+}
+__host__ __device__ static inline double amc_exp_dd_round(amc_dd z) {
+  const double k = rint(z.hi * 1.4426950408889634);  // z / ln 2
+  amc_dd r = amc_dd_add(z, amc_dd_neg(amc_dd_mul_d(amc_tab(kLn2, 0), k)));
+  const int j = (int)rint(r.hi * AMC_DD_TAB);         // -23 .. 23
+  r = amc_dd_add(r, amc_dd{-(double)j / AMC_DD_TAB, 0.0});
+  const amc_dd er = amc_poly(kExpCoef, 13, r);
+  const amc_dd v = amc_dd_mul(amc_tab(kExpTab, j + 23), er);
+  return ldexp(v.hi + v.lo, (int)k);
+}
+__host__ __device__ static inline double amc_pow_cr(double x, double y) {
+  if (y == 0.0) return 1.0;
+  if (y == 1.0) return x;
+  return amc_exp_dd_round(amc_dd_mul_d(amc_log_dd(x), y));
+}

@@ -172,11 +172,17 @@ def full_run(deck):
     info = sim.info()
     nvox = info["nX"][0] * info["nX"][1] * info["nX"][2]
     t0 = time.perf_counter()
-    steps = sim.run()
+    steps, planes, last = 0, [], None
+    while sim.step() > 0:
+        steps += 1
+        inf = sim.info()
+        if inf["ilaserLoc"] != last:   # a new layer: (step, laser plane, grid rotation)
+            planes.append((inf["tInd"], inf["ilaserLoc"], round(inf["gth"], 6)))
+            last = inf["ilaserLoc"]
+    steps += 1
     wall = time.perf_counter() - t0
-    layers = sim.info()["indlayer"] - 1
     sim.close()
-    return {"steps": steps, "layers": layers, "wall_s": round(wall, 3), "ms_per_step": round(1e3 * wall / steps, 4),
+    return {"steps": steps, "layers": planes, "wall_s": round(wall, 3), "ms_per_step": round(1e3 * wall / steps, 4),
             "value": round(nvox * steps / wall, 1), "unit": "cell-updates/s"}
 
 

@@ -101,8 +101,8 @@ int amcafe_analytic_temp_curr(amcafe_domain* d, const amcafe_laser* laser, int* 
  * above the laser plane, 0.99 T0 ahead of the laser in the top layer (layerT).  *all_melt: every
  * voxel at T >= tL (the end-of-scan test of S/TempField.C:117-126); *n_integrals: source
  * integrals evaluated.  Either may be NULL.  The reference cannot select this model from a deck
- * (its EASM keyword never matches, S/Grid.C:198-199,276); UpdateVoxels after it reports
- * AMCAFE_ERR_DEVICE, since growth velocities exist only for the analytic temperatures. */
+ * (its EASM keyword never matches, S/Grid.C:198-199,276).  amcafe_update_voxels after it grows
+ * every cell at its own temperature, vhat = Avel (tL - T)^nvel with pow rounded to nearest. */
 int amcafe_easm_temp_curr(amcafe_domain* d, const amcafe_laser* laser, double bmP, double layerT,
                           int* all_melt, long long* n_integrals);
 /* VoxelsCA::UpdateVoxels on the current TempCurr; tInd = Grid::tInd. */

@@ -24,8 +24,9 @@
 // the same third-party library the reference links.
 //
 // Math backend: by default glibc libm (what the reference calls).  With
-// orc_set_crmath(1) the four transcendental call sites of the capture geometry
-// (atan2, cos, sin, pow(x,0.5)) use correctly rounded results from libquadmath
+// orc_set_crmath(1) the transcendental call sites of the capture loop (atan2, cos, sin,
+// pow(x,0.5) of the geometry and pow(tL - T, nvel) of the growth velocity) use correctly
+// rounded results from libquadmath
 // instead; that mode is a bit-exact model of the GPU kernels' own math and is
 // used to measure how much libm rounding matters for the decisions.
 
@@ -56,6 +57,8 @@ static double m_sin(double x) { return g_crmath ? (double)sinq((__float128)x) : 
 // pow(x,.5) is a libm call in the reference (GCC does not fold it to sqrt without
 // -funsafe-math-optimizations); sqrt is its correctly rounded value.
 static double m_powh(double x) { return g_crmath ? std::sqrt(x) : std::pow(x, .5); }
+// pow(tL - T, nvel) of the growth velocity (S/VoxelsCA.C:405)
+static double m_pow(double x, double y) { return g_crmath ? (double)powq((__float128)x, (__float128)y) : std::pow(x, y); }
 
 // pow(cos th * sin ph, .5) of S/VoxelsCA.C:415-418, 424 (also :548-551, 591), a pure
 // function of the centre-to-centroid vector; memoised because the reference's scan
@@ -819,7 +822,7 @@ struct Sim {
         }
         if (!(anyliq && allcold)) continue;
         grain_rot(G[i1], R);
-        T[i1] >= tL ? vhatvec[j] = 0.0 : vhatvec[j] = g.Avel * pow(tL - T[i1], g.nvel);
+        T[i1] >= tL ? vhatvec[j] = 0.0 : vhatvec[j] = g.Avel * m_pow(tL - T[i1], g.nvel);
         for (int j1 = 0; j1 < i6; ++j1) {
           int i4 = nidx[i5 + j1];
           if (vS[i4] != 1) continue;
@@ -1281,6 +1284,10 @@ void orc_rot_mat(double omega, const double* ax, double* R9) {
   for (int i = 0; i < 9; ++i) R9[i] = R[i / 3][i % 3];
 }
 void orc_shape_factor_n(const double* dnx, int n, double* out);
+// the growth velocity's pow in the current math mode
+void orc_pow_n(const double* x, const double* y, int n, double* out) {
+  for (int i = 0; i < n; ++i) out[i] = orc::m_pow(x[i], y[i]);
+}
 double orc_shape_factor(const double* dnx) { return orc::shape_factor_raw(dnx); }
 void orc_shape_factor_n(const double* dnx, int n, double* out) {
   for (int i = 0; i < n; ++i) out[i] = orc_shape_factor(dnx + 3 * i);

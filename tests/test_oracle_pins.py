@@ -343,3 +343,47 @@ def test_easm_regions_and_far_field():
     T = f["TempCurr"].reshape(nz, ny, nx)
     assert T[il - 1, ny // 2, nx // 2 - 1] > 1733.0
     assert np.all(f["BuildjID"][f["TempCurr"] > 1733.0] == 1)
+
+
+# ---------------------------------------------------------------------------------------
+# growth velocity pow(tL - T, nvel) (S/VoxelsCA.C:405)
+def _pow(L, name, x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(len(x))
+    f = getattr(L, name)
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    f(x.ctypes.data, y.ctypes.data, len(x), out.ctypes.data)
+    return out
+
+
+def test_device_pow_is_correctly_rounded():
+    """The device's double-double pow (ddmath.h, host build) against quadmath's powq rounded
+    to double, on undercoolings tL - T of any temperature an EASM field gives (0 K to tL),
+    exponents around the decks' 3, exact cases and values near 1."""
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(1e-6, 1733.0, 20000), 10 ** rng.uniform(-8, 3.3, 20000),
+                        1 + rng.normal(size=2000) * 1e-9, [2.0, 10.0, 1733.0, 0.5, 1.0, 27.144, 29.24]])
+    y = np.concatenate([np.full(20000, 3.0), rng.uniform(0.5, 4.0, 20000), np.full(2000, 3.0),
+                        [3.0, 3.0, 3.0, 3.0, 3.0, 3.0, 3.0]])
+    got = _pow(_product_lib(), "amcd_host_pow_n", x, y)
+    want = _pow(oracle_lib(True), "orc_pow_n", x, y)
+    assert np.array_equal(got, want)
+    assert got[-7] == 8.0 and got[-6] == 1000.0 and got[-3] == 1.0
+
+
+@pytest.mark.parametrize("deck_name", ["t_small.in", "t_odd.in", "c0_single_track.in", "c1_raster.in",
+                                       "c4_production.in"] + [f"c2_batched/inst_{i:02d}.in" for i in range(64)])
+def test_analytic_velocities_agree_in_both_math_modes(deck_name):
+    """The analytic model's growing cells sit at tS or T0; for every committed deck glibc's
+    pow gives the correctly rounded value there, so the device's velocities equal the
+    reference arithmetic's exactly (the bit-exact bar against the glibc oracle relies on it)."""
+    kv = {}
+    for line in read(deck_name).split("\n"):
+        p = line.split("#")[0].split()
+        if p:
+            kv[p[0].lower()] = p[1:]
+    tL, tS, nvel = float(kv["tl"][0]), float(kv["ts"][0]), float(kv["dendritegrowth2"][1])
+    x = np.array([tL - tS, tL - 300.0])
+    assert np.array_equal(_pow(oracle_lib(False), "orc_pow_n", x, [nvel, nvel]),
+                          _pow(_product_lib(), "amcd_host_pow_n", x, [nvel, nvel]))
