@@ -7,3 +7,4 @@ with ctypes; it has no compute path of its own and raises if the library is abse
 from ._lib import lib, LibraryMissing, AmcafeError  # noqa: F401
 from .sim import Sim, Domain, read_deck  # noqa: F401
 from .heat import HeatBatch, HeatField, calibrate, track_desc  # noqa: F401
+from .meltpool import production_deck, semi_axes  # noqa: F401

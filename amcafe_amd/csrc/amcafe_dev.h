@@ -108,6 +108,7 @@ int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, cons
 // (theta, target, grad: 4 per instance; loss: one per instance): mode 0 loss (FP32), 1 loss
 // and gradient (FP32), 2 loss (FP64); amcafe_heat_desc is declared in include/amcafe.h
 struct amcafe_heat_desc_s;
+unsigned long long amcd_heat_last_evals(void);
 int amcd_heat(const struct amcafe_heat_desc_s* d, int nb, const double* theta, const double* target, int step0,
               int nsteps, int mode, double* loss, double* grad, float* ms, char* err, int errlen);
 // EASM temperature (S/TempField.C:79-137) instead of the analytic one: beam power Q,

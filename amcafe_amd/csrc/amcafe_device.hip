@@ -1207,9 +1207,9 @@ __device__ static inline Best best_none() {
   return b;
 }
 // a strictly before b in (t, ruqs, scan order).  The ordering predicates are written
-// without short-circuit branches: hipcc (ROCm 7.2, gfx950, -O3) miscompiles
-// `if (better(o, b)) b = o;` when better() returns early from a chain of comparisons,
-// leaving some fields of b from the old value and some from o.
+// without short-circuit branches, and a winner is taken field by field with selects on one
+// predicate value, so that every field of the result comes from the same operand whatever
+// the compiler does with the comparison chain.
 __device__ static inline bool better(const Best& a, const Best& b) {
   const bool tl = a.t < b.t, tg = a.t > b.t, rl = a.r < b.r, rg = a.r > b.r;
   return tl | (!tg & (rl | (!rg & (a.key < b.key))));
@@ -1243,6 +1243,15 @@ __device__ static inline float wave_min(float v) {
   v = fminf(v, dpp<kBcast31, 0xc>(v));
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
+__device__ static inline double wave_min_f64(double v) {
+  v = fmin(v, dpp<kQuadX1>(v));
+  v = fmin(v, dpp<kQuadX2>(v));
+  v = fmin(v, dpp<kHalfMirror>(v));
+  v = fmin(v, dpp<kMirror>(v));
+  v = fmin(v, dpp<kBcast15, 0xa>(v));
+  v = fmin(v, dpp<kBcast31, 0xc>(v));
+  return lane_f64(v, 63);
+}
 template <int CTRL, int ROWS = 0xf>
 __device__ static inline void best_step(Best& b) {
   Best o;
@@ -1268,11 +1277,16 @@ __device__ static inline Best lane_best(const Best& b, int l) {
   return o;
 }
 // best over a wave.  (t, ruqs, key) is a strict total order on candidates -- keys are
-// unique -- so the order in which lanes are combined does not change the result.
+// unique -- so the order in which lanes are combined does not change the result.  A
+// single candidate, or a single one at the smallest time, is the best without the full
+// comparison.
 __device__ static inline Best wave_best(Best b) {
   const unsigned long long m = __ballot(b.cand >= 0);
   if (m == 0) return best_none();
   if (__popcll(m) == 1) return lane_best(b, __builtin_ctzll(m));
+  const double tmin = wave_min_f64(b.cand >= 0 ? b.t : __builtin_inf());
+  const unsigned long long at = __ballot((b.cand >= 0) & (b.t == tmin));
+  if (__popcll(at) == 1) return lane_best(b, __builtin_ctzll(at));
   best_step<kQuadX1>(b);
   best_step<kQuadX2>(b);
   best_step<kHalfMirror>(b);
@@ -1509,17 +1523,32 @@ __device__ static inline Decision decide_from(const EngineShared& sh, Best e, in
   int wi = lane;
   const double U = sh.ubuf[(nd >> 6) & 1][nd & 63];
   const int susp = sh.susp[par];
-  // pairwise tree, the left entry kept unless the right one is strictly better: the
-  // first best entry in wave order, as a sequential scan finds it (the order is a
-  // strict weak order: times and ruqs are never NaN here)
-  tournament_level<NW, 1>(e, wi);
+  // The smallest time first (three row shifts of one double): when a single entry holds it,
+  // that entry is the best whatever its ruqs and key.  Otherwise the pairwise tree, the
+  // left entry kept unless the right one is strictly better: the first best entry in wave
+  // order, as a sequential scan finds it (the order is a strict weak order: times and ruqs
+  // are never NaN here).
+  double tm = e.t;
+  tm = fmin(tm, row_from<1>(tm));
+  tm = fmin(tm, row_from<2>(tm));
+  tm = fmin(tm, row_from<4>(tm));
+  static_assert(NW <= 8, "three levels cover the entries");
+  const double t0 = lane_f64(tm, 0);
+  const unsigned long long at_min = __ballot((lane < NW) & (e.cand >= 0) & (e.t == t0));
+  int wq;
   Best b;
-  b.t = lane_f64(e.t, 0);
-  b.r = lane_f64(e.r, 0);
-  b.key = __builtin_amdgcn_readlane(e.key, 0);
-  b.cand = __builtin_amdgcn_readlane(e.cand, 0);
-  b.tgt = __builtin_amdgcn_readlane(e.tgt, 0);
-  const int wq = __builtin_amdgcn_readlane(wi, 0);
+  if (__popcll(at_min) == 1) {
+    wq = __builtin_ctzll(at_min);
+    b = lane_best(e, wq);
+  } else {
+    tournament_level<NW, 1>(e, wi);
+    b.t = lane_f64(e.t, 0);
+    b.r = lane_f64(e.r, 0);
+    b.key = __builtin_amdgcn_readlane(e.key, 0);
+    b.cand = __builtin_amdgcn_readlane(e.cand, 0);
+    b.tgt = __builtin_amdgcn_readlane(e.tgt, 0);
+    wq = __builtin_amdgcn_readlane(wi, 0);
+  }
   // branch-free, so that the reads of the uniform and the suspension flag are issued
   // with the read of the bests and not one after another
   Decision x;

@@ -614,11 +614,15 @@ extern "C" int amcafe_ipf_colours(const double* cTheta, int nGrain, float* rgb9)
 
 // ====================================================================================
 // 3. Differentiable temperature field
+static bool heat_args_ok(const amcafe_heat_desc* d, int nb, const double* theta, const double* target, int step0,
+                         int nsteps, const double* loss) {
+  return d && nb > 0 && nb <= 65535 && theta && target && loss && nsteps >= 0 && step0 >= 0 && d->nX[0] > 0 &&
+         d->nX[1] > 0 && d->nX[2] > 0 && (long long)d->nX[0] * d->nX[1] * d->nX[2] <= 0x7fffffffLL && d->dX > 0 &&
+         d->dt > 0 && d->t_hist > 0 && d->width > 0 && d->k > 0 && d->alpha > 0;
+}
 static int heat(const amcafe_heat_desc* d, int nb, const double* theta, const double* target, int step0, int nsteps,
                 int mode, double* loss, double* grad, float* ms) {
-  if (!d || nb <= 0 || nb > 65535 || !theta || !target || !loss || nsteps < 0 || step0 < 0 || d->nX[0] <= 0 ||
-      d->nX[1] <= 0 || d->nX[2] <= 0 || (long long)d->nX[0] * d->nX[1] * d->nX[2] > 0x7fffffffLL || !(d->dX > 0) ||
-      !(d->dt > 0) || !(d->t_hist > 0) || !(d->width > 0) || !(d->k > 0) || !(d->alpha > 0) || (mode == 1 && !grad))
+  if (!heat_args_ok(d, nb, theta, target, step0, nsteps, loss) || (mode == 1 && !grad))
     return fail(AMCAFE_ERR_ARG, "bad heat-field arguments");
   char err[256] = {0};
   if (amcd_heat(d, nb, theta, target, step0, nsteps, mode, loss, grad, ms, err, sizeof err))
@@ -637,6 +641,7 @@ extern "C" int amcafe_heat_loss_f64(const amcafe_heat_desc* d, const double* the
                                     int nsteps, double* loss, float* ms) {
   return heat(d, 1, theta, target, step0, nsteps, 2, loss, nullptr, ms);
 }
+extern "C" long long amcafe_heat_last_evaluations(void) { return (long long)amcd_heat_last_evals(); }
 extern "C" int amcafe_heat_loss_grad_batch_f32(const amcafe_heat_desc* d, int n, const double* theta,
                                                const double* target, int step0, int nsteps, double* loss, double* grad,
                                                float* ms) {

@@ -72,10 +72,23 @@ def algorithmic_bytes(stage, nvox, st):
     return None
 
 
+def _digest(sim, i):
+    """SHA-256 of an instance's fields and grain table (outside every timed region)."""
+    import hashlib
+    h = hashlib.sha256()
+    f = sim.fields(i)
+    for key in sorted(f):
+        h.update(key.encode())
+        h.update(np.ascontiguousarray(f[key]).tobytes())
+    h.update(sim.grains(i).tobytes())
+    return h.hexdigest()
+
+
 def batched_vs_single(deck_dir):
     """The batched workload: every deck of deck_dir run to completion in one Sim (one launch per
     stage for all instances), then each deck alone in its own Sim, one after the other.  Host
-    set-up (deck parse, base plate, first layer) is outside both timed regions."""
+    set-up (deck parse, base plate, first layer) is outside both timed regions; so is the
+    comparison of every instance's final fields and grain table between the two."""
     import glob
     import amcafe_amd
     names = sorted(glob.glob(os.path.join(deck_dir, "*.in")))
@@ -91,22 +104,25 @@ def batched_vs_single(deck_dir):
     bsteps = sim.run()
     tb = time.perf_counter() - t0
     upd = sum(nv[i] * (sim.info(i)["tInd"] - t0i[i]) for i in range(len(decks)))
+    want = [_digest(sim, i) for i in range(len(decks))]
     sim.close()
-    ts, sup = 0.0, 0
-    for d in decks:
+    ts, sup, same = 0.0, 0, 0
+    for i, d in enumerate(decks):
         one = amcafe_amd.Sim(d)
         inf = one.info(0)
         t0 = time.perf_counter()
         one.run()
         ts += time.perf_counter() - t0
         sup += inf["nX"][0] * inf["nX"][1] * inf["nX"][2] * (one.info(0)["tInd"] - inf["tInd"])
+        same += int(_digest(one, 0) == want[i])
         one.close()
     assert sup == upd
     return {"workload": os.path.basename(deck_dir.rstrip("/")), "instances": len(decks), "voxels": int(sum(nv)),
             "cell_updates": int(upd), "batched_steps": bsteps,
             "batched_value": round(upd / tb, 1), "batched_s": round(tb, 3),
             "sequential_value": round(sup / ts, 1), "sequential_s": round(ts, 3),
-            "speedup": round(ts / tb, 3), "unit": "cell-updates/s"}
+            "speedup": round(ts / tb, 3), "unit": "cell-updates/s",
+            "instances_bit_identical_to_single_runs": same}
 
 
 def adjoint_cost(fd_steps=2000):
@@ -115,6 +131,7 @@ def adjoint_cost(fd_steps=2000):
     (P, v, eta, sigma) = (220 W, 0.9 m/s, 0.4, 45 um), evaluated at (200, 0.8, 0.35, 50 um):
     FP32 forward time, FP32 forward + adjoint time, and the adjoint gradient against FP64
     central differences (relative step 1e-4) over the first fd_steps steps."""
+    import amcafe_amd
     from amcafe_amd.heat import HeatField, track_desc
     target, start = (220.0, 0.9, 0.4, 45e-6), (200.0, 0.8, 0.35, 50e-6)
     f = HeatField(track_desc((256, 256, 64), steps=2000, checkpoint=100), target)
@@ -122,16 +139,27 @@ def adjoint_cost(fd_steps=2000):
     t0 = time.perf_counter()
     loss, ms_f = f.loss_f32(start, 0, 2000)
     tf = time.perf_counter() - t0
+    evals = int(amcafe_amd.lib().amcafe_heat_last_evaluations())
     t0 = time.perf_counter()
     loss_g, grad, ms_g = f.loss_grad_f32(start, 0, 2000)
     tg = time.perf_counter() - t0
+    again = f.loss_grad_f32(start, 0, 2000)
+    reproducible = again[0] == loss_g and again[1] == grad
     fd = f.central_differences(start, 0, fd_steps, h=1e-4)
     if fd_steps != 2000:
         _, grad, _ = f.loss_grad_f32(start, 0, fd_steps)
     rel = [abs(g - d) / abs(d) for g, d in zip(grad, fd)]
+    # FP32 operations per evaluated voxel-step, counted in the kernel's source: per quadrature
+    # node 6 for the field and 6 for the target's (x offset, square, scale, exp, weight, sum),
+    # 10 more for the two gradient sums; 25 nodes, plus the indicator and accumulation per step
+    ops_f, ops_g = 25 * 12 + 12, 25 * 22 + 20
     return {"workload": "c3_heat", "voxels": 256 * 256 * 64, "steps": 2000, "dtype": "f32",
             "loss": loss, "forward_ms": round(ms_f, 2), "forward_and_adjoint_ms": round(ms_g, 2),
             "cost_ratio": round(ms_g / ms_f, 3), "wall_forward_s": round(tf, 3), "wall_adjoint_s": round(tg, 3),
+            "evaluated_voxel_steps": evals, "all_voxel_steps": 256 * 256 * 64 * 2000,
+            "forward_tflops": round(evals * ops_f / (ms_f * 1e-3) / 1e12, 2),
+            "adjoint_tflops": round(evals * ops_g / (ms_g * 1e-3) / 1e12, 2), "fp32_peak_tflops": 157.3,
+            "bit_reproducible": bool(reproducible),
             "grad": grad, "fd_grad": fd, "fd_steps": fd_steps, "max_rel_err_vs_fd": max(rel)}
 
 
@@ -272,13 +300,22 @@ def main():
     if abytes is not None:
         per_launch = abytes / tsteps
         achieved = per_launch / (launch_ms * 1e-3) / 1e9
-        # HBM bytes per launch are measured only by rocprofv3 counter runs, which cannot run
-        # inside this process; the summaries of such runs are kept beside the kernel traces
+        # HBM bytes per launch come from rocprofv3 counter runs, which cannot run inside this
+        # process: a committed measurement is used only when it was taken on this workload over
+        # the same steps (the kernel's dispatches of the stage-timed steps of this command)
+        window = f"steps {first_timed + steps}-{first_timed + steps + tsteps - 1}"
+        traffic, tsrc = None, None
+        tfile = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{args.workload}.json")
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                tj = json.load(f)
+            if tj.get("window") == window and tj.get("workload") == args.workload:
+                traffic = tj.get("hbm_bytes_per_launch")
+                tsrc = os.path.relpath(tfile, ROOT)
         roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": KERNEL_NAMES[dom], "launch_ms": round(launch_ms, 4),
-                "algorithmic_bytes_per_launch": round(per_launch, 1),
-                "window": f"steps {first_timed + steps}-{first_timed + steps + tsteps - 1} (HIP events)"}
+                "algorithmic_bytes_per_launch": round(per_launch, 1), "window": window + " (HIP events)"}
     ev = tot["n_events"] / tsteps
     cap_us_per_event = (stage_ms[4] / tsteps) * 1e3 / ev if ev > 0 else None
 

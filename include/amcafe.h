@@ -192,10 +192,16 @@ int amcafe_heat_loss_f64(const amcafe_heat_desc* d, const double* theta, const d
                          int nsteps, double* loss, float* ms);
 /* n instances on the same grid and track in one launch per checkpoint segment (the calibration
  * form): theta, target and grad hold 4 values per instance, loss one (1 <= n <= 65535).  Each
- * instance's result is that of amcafe_heat_loss_grad_f32 on its own parameters, up to the order
- * of the FP32 block sums. */
+ * instance's result is bit-identical to amcafe_heat_loss_grad_f32 on its own parameters.
+ * All FP32 entry points sum in a fixed order (per thread over steps, per 256-voxel block in a
+ * fixed tree, blocks across segments in launch order, block slots in a fixed tree), so a call
+ * repeated with the same arguments returns the same bits.  They skip voxels whose temperature
+ * bound stays below tL - 40 width at every step for both parameter sets (indicator below e^-40,
+ * loss term below 2e-35); the FP64 forward evaluates every voxel. */
 int amcafe_heat_loss_grad_batch_f32(const amcafe_heat_desc* d, int n, const double* theta, const double* target,
                                     int step0, int nsteps, double* loss, double* grad, float* ms);
+/* voxel-steps (per instance) the last heat call of this thread evaluated, after pruning */
+long long amcafe_heat_last_evaluations(void);
 
 #ifdef __cplusplus
 }

@@ -472,20 +472,33 @@ struct Sim {
     return 0;
   }
 
-  // S/BasePlate.C:31-97 + S/VoxelsCA.C:125-161
-  void initialize_voxels() {
+  // the base plate's Poisson grain count and Voronoi sites (S/BasePlate.C:31-58), drawn from
+  // the member engine
+  int draw_plate_sites(std::vector<double>& Xs) {
     double rate = g.mu * (g.dX[0] * 1e6) * (g.dX[1] * 1e6) * g.nX[0] * g.nX[1] * (bp_height * 1e6);
     std::poisson_distribution<int> ng(rate);
     int Ngrain = 0;
     while (Ngrain == 0) Ngrain = ng(bp_generator);
     std::uniform_real_distribution<double> xrand(0.0, 1.0);
     double Lx = g.dX[0] * g.nX[0], Ly = g.dX[1] * g.nX[1];
-    std::vector<double> Xs(3 * (size_t)Ngrain);
+    Xs.assign(3 * (size_t)Ngrain, 0.0);
     for (int j = 0; j < Ngrain; ++j) {  // braced list: evaluated left to right
       Xs[3 * j] = xrand(bp_generator) * Lx;
       Xs[3 * j + 1] = xrand(bp_generator) * Ly;
       Xs[3 * j + 2] = xrand(bp_generator) * bp_height;
     }
+    return Ngrain;
+  }
+  // the plate's orientation seed (S/BasePlate.C:83-86)
+  static unsigned int plate_sdloc() {
+    Engine g1(531231);
+    return unsigned(double(g1()) / double(g1.max()) * pow(2.0, 32.0));
+  }
+
+  // S/BasePlate.C:31-97 + S/VoxelsCA.C:125-161
+  void initialize_voxels() {
+    std::vector<double> Xs;
+    const int Ngrain = draw_plate_sites(Xs);
     last_sites = Xs;
     int nplate = Nzh * g.nX[0] * g.nX[1];
     std::vector<int> X(nplate, 0);
@@ -501,8 +514,7 @@ struct Sim {
       }
       X[j] = best + 1;
     }
-    Engine g1(531231);
-    unsigned int sdloc = unsigned(double(g1()) / double(g1.max()) * pow(2.0, 32.0));
+    unsigned int sdloc = plate_sdloc();
     std::vector<double> aa;
     sa.generate(Ngrain, sdloc, aa);
     bpNgrain = Ngrain;
@@ -978,16 +990,18 @@ struct Sim {
       if (gID[j] != 0) gID[j] = remap[gID[j] - 1];
   }
 
-  void add_layer1() {  // S/VoxelsCA.C:887-982
+  // AddLayer1's orientation seed, Poisson site count and sites (S/VoxelsCA.C:895-919), drawn
+  // from the layer engine; returns the site count
+  int draw_layer_sites(std::vector<double>& Xs, unsigned int& sdloc) {
     const int nx = g.nX[0], ny = g.nX[1];
-    unsigned int sdloc = unsigned(double(genlayer()) / double(genlayer.max()) * pow(2.0, 32.0));
+    sdloc = unsigned(double(genlayer()) / double(genlayer.max()) * pow(2.0, 32.0));
     double rate = g.lrate * (g.dX[0] * 1e6) * (g.dX[1] * 1e6) * nx * ny * (g.layerT * 1e6);
     std::poisson_distribution<int> ngd(rate);
     int ng2 = 0;
     while (ng2 == 0) ng2 = ngd(genlayer);
     int iz1 = g.ilaserLoc - g.nZlayer;
     double zloc0 = iz1 * g.dX[2];
-    std::vector<double> Xs(ng2 * 3);
+    Xs.assign(ng2 * 3, 0.0);
     std::uniform_real_distribution<double> xrand(0.0, 1.0);
     double Lx = nx * g.dX[0], Ly = ny * g.dX[1];
     for (int j = 0; j < ng2; ++j) {
@@ -995,6 +1009,15 @@ struct Sim {
       Xs[3 * j + 1] = xrand(genlayer) * Ly;
       Xs[3 * j + 2] = xrand(genlayer) * g.layerT + zloc0;
     }
+    return ng2;
+  }
+
+  void add_layer1() {  // S/VoxelsCA.C:887-982
+    const int nx = g.nX[0], ny = g.nX[1];
+    unsigned int sdloc = 0;
+    std::vector<double> Xs;
+    const int ng2 = draw_layer_sites(Xs, sdloc);
+    int iz1 = g.ilaserLoc - g.nZlayer;
     last_sites = Xs;
     const int nGrain_before = nGrain;
     int i3 = nx * ny;
@@ -1252,6 +1275,35 @@ long orc_last_sites(void* h, double* xs) {
   if (xs) std::copy(s->last_sites.begin(), s->last_sites.end(), xs);
   return (long)(s->last_sites.size() / 3);
 }
+// The initial state's random draws of a deck, without the Voronoi labelling (for checks at
+// sizes where the oracle's brute-force labelling is too slow): the base plate's sites and
+// orientation seed, then the first powder layer's sites and orientation seed.  counts gets
+// {plate sites, layer sites, Nzh, first layer's iz1, first layer's ilaserLoc}, seeds gets
+// {plate seed, layer seed}.  Arrays may be null to query the counts.
+void orc_initial_sites(const char* deck, double* plate, double* layer, long* counts, unsigned int* seeds) {
+  orc::Sim s;
+  std::string err;
+  if (s.setup(deck, err, true)) {
+    counts[0] = -1;
+    return;
+  }
+  std::vector<double> xp, xl;
+  const int ng = s.draw_plate_sites(xp);
+  unsigned int sdl = 0;
+  const int ng2 = s.draw_layer_sites(xl, sdl);
+  counts[0] = ng;
+  counts[1] = ng2;
+  counts[2] = s.Nzh;
+  counts[3] = s.g.ilaserLoc - s.g.nZlayer;
+  counts[4] = s.g.ilaserLoc;
+  if (seeds) {
+    seeds[0] = orc::Sim::plate_sdloc();
+    seeds[1] = sdl;
+  }
+  if (plate) std::copy(xp.begin(), xp.end(), plate);
+  if (layer) std::copy(xl.begin(), xl.end(), layer);
+}
+
 // the last AddLayer1: iz1, ilaserLoc, sites drawn, labels used, orientation seed, nGrain before
 void orc_last_layer(void* h, long* out6) {
   const orc::Sim* s = (const orc::Sim*)h;

@@ -59,6 +59,7 @@ def _load(path):
         getattr(L, f).argtypes = [_P]
     L.orc_convert_solid.argtypes = [_P, _I]
     L.orc_set_build_flags.argtypes = [_P, _P]
+    L.orc_initial_sites.argtypes = [ctypes.c_char_p, _P, _P, _P, _P]
     return L
 
 
@@ -267,3 +268,38 @@ def tiny_deck(nX=(6, 5, 4), **over):
         kv[k] = v if isinstance(v, str) else " ".join(repr(float(x)) for x in v) if hasattr(v, "__len__") \
             else repr(v)
     return "\n".join(f"{k} {v}" for k, v in kv.items()) + "\n"
+
+
+def initial_sites(deck_text):
+    """The initial state's random draws of a deck, without the Voronoi labelling: (plate sites
+    (n, 3), plate orientation seed, first layer sites (m, 3), layer orientation seed, Nzh,
+    first layer's iz1, its ilaserLoc)."""
+    L = oracle_lib(False)
+    c = np.zeros(5, np.int64)
+    seeds = np.zeros(2, np.uint32)
+    L.orc_initial_sites(deck_text.encode(), None, None, c.ctypes.data, seeds.ctypes.data)
+    if c[0] < 0:
+        raise RuntimeError(L.orc_last_error().decode())
+    xp, xl = np.zeros(3 * c[0]), np.zeros(3 * c[1])
+    L.orc_initial_sites(deck_text.encode(), xp.ctypes.data, xl.ctypes.data, c.ctypes.data, seeds.ctypes.data)
+    return xp.reshape(-1, 3), int(seeds[0]), xl.reshape(-1, 3), int(seeds[1]), int(c[2]), int(c[3]), int(c[4])
+
+
+def samples(n, seed):
+    """GenerateSamples(n, seed) of the oracle's sampler (pinned against the reference build)."""
+    out = np.zeros(4 * n)
+    oracle_lib(False).orc_generate_samples(n, seed, out.ctypes.data)
+    return out
+
+
+def voronoi_labels(points, sites):
+    """0-based index of the nearest site of each point, with the reference's squared distance
+    ax*ax + ay*ay + az*az and its first-minimum rule: a k-d tree proposes the two nearest
+    sites, which are compared with that expression (ties go to the lower index)."""
+    from scipy.spatial import cKDTree
+    _, idx = cKDTree(sites).query(points, k=2)
+    d = [((points - sites[idx[:, q]]) ** 2) for q in range(2)]
+    d = [(e[:, 0] + e[:, 1]) + e[:, 2] for e in d]
+    a, b = idx[:, 0], idx[:, 1]
+    pick_b = (d[1] < d[0]) | ((d[1] == d[0]) & (b < a))
+    return np.where(pick_b, b, a)

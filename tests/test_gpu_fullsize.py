@@ -1,0 +1,134 @@
+"""Full-size checks of the configurations whose oracle runs are too slow to follow from the
+first step:
+  * the initial state of c1 (512x512x128, 2000-grain plate) and of the production run
+    (1024x512x128, 5000-grain plate): the base plate's labels at voxel centres on a random
+    sample of plate voxels and the first powder layer's labels at voxel corners on every layer
+    voxel, against a nearest-site search over the oracle's own site draws; states, the
+    renumbering of the used sites and both orientation tables;
+  * c1 handed to both oracles after its first layer change (the laser one layer up, the grid
+    turned by 90 deg, a fresh powder layer), twice: two steps after the change, and 60 steps
+    into the new layer with the melt pool established; the next two steps field for field;
+  * the 64 decks of the batched workload run as one batch and one by one: every instance's
+    fields and grain table bit-identical.
+"""
+import glob
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import amcafe_amd
+from oracle_util import ROOT, Oracle, initial_sites, read, samples, voronoi_labels
+
+pytestmark = pytest.mark.gpu
+DX = 2.5e-6
+
+
+def _same(a, b):
+    return np.array_equal(a, b, equal_nan=a.dtype.kind == "f")
+
+
+@pytest.mark.parametrize("deck_name", ["c1_raster.in", "c4_production.in"])
+def test_full_size_initial_state(deck_name):
+    deck = read(deck_name)
+    sim = amcafe_amd.Sim(deck)
+    nx, ny, nz = sim.info(0)["nX"]
+    nxy = nx * ny
+    f = sim.fields(0, which=("gID", "vState"))
+    ct = sim.grains(0).reshape(-1, 4)
+    xp, sdp, xl, sdl, nzh, iz1, il = initial_sites(deck)
+    ng = len(xp)
+    print(f"{deck_name}: {ng} plate grains, {len(xl)} layer sites")
+    # base plate: solid below its top plane, mushy in it; labels at voxel centres (sampled)
+    vs, gid = f["vState"], f["gID"]
+    assert np.all(vs[:(nzh - 1) * nxy] == 3) and np.all(vs[(nzh - 1) * nxy:nzh * nxy] == 2)
+    rng = np.random.default_rng(3)
+    v = rng.integers(0, nzh * nxy, 200000)
+    k, r = np.divmod(v, nxy)
+    j, i = np.divmod(r, nx)
+    pts = np.stack([(i + .5) * DX, (j + .5) * DX, (k + .5) * DX], axis=1)
+    assert np.array_equal(gid[v], voronoi_labels(pts, xp) + 1)
+    assert np.array_equal(ct[:ng].ravel(), samples(ng, sdp))
+    # first powder layer: every voxel, corners, used sites renumbered after the plate's grains
+    assert iz1 == nzh
+    v = np.arange(iz1 * nxy, il * nxy)
+    k, r = np.divmod(v, nxy)
+    j, i = np.divmod(r, nx)
+    lab = voronoi_labels(np.stack([i * DX, j * DX, k * DX], axis=1), xl)
+    used = np.unique(lab)
+    rank = np.full(len(xl), -1)
+    rank[used] = np.arange(len(used))
+    assert np.array_equal(gid[iz1 * nxy:il * nxy], ng + 1 + rank[lab])
+    assert np.all(vs[iz1 * nxy:il * nxy] == 3)
+    assert np.all(vs[il * nxy:] == 0) and np.all(gid[il * nxy:] == 0)
+    assert len(ct) == ng + len(used) + 1
+    assert np.array_equal(ct[ng:].ravel(), samples(len(used) + 1, sdl))
+
+
+def _handover(sim, deck, label, min_events):
+    """Two steps of the device against both oracles from the device's current state."""
+    info = sim.info(0)
+    f = sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID"))
+    cr, gl = Oracle(deck, crmath=True, bare=True), Oracle(deck, crmath=False, bare=True)
+    for o in (cr, gl):
+        o.load_state(f, sim.grains(0), info)
+    events = 0
+    for m in range(2):
+        sim.step()
+        for o in (cr, gl):
+            o.temperature()
+            o.update_voxels()
+            o.finish_step()
+        g, c, lg = sim.fields(0), cr.fields(), gl.fields()
+        for key in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
+            assert _same(g[key], c[key]), f"{label} +{m + 1}: {key}"
+        assert np.array_equal(g["gID"], lg["gID"]) and np.array_equal(g["vState"], lg["vState"]), label
+        np.testing.assert_allclose(g["extents"], lg["extents"], rtol=1e-12, atol=1e-21)
+        assert sim.stats(0)["n_events"] == cr.stats()["n_events"] == gl.stats()["n_events"]
+        events += sim.stats(0)["n_events"]
+    assert np.array_equal(sim.grains(0), cr.grains())
+    assert events >= min_events, (label, events)
+
+
+def test_c1_handover_after_the_first_layer_change():
+    deck = read("c1_raster.in")
+    sim = amcafe_amd.Sim(deck)
+    il0 = sim.info(0)["ilaserLoc"]
+    n = 0
+    while sim.info(0)["ilaserLoc"] == il0:
+        assert sim.step() > 0
+        n += 1
+    for _ in range(2):
+        sim.step()
+    info = sim.info(0)
+    assert info["ilaserLoc"] == il0 + 12 and abs(info["gth"] - np.pi) < 1e-12
+    _handover(sim, deck, f"c1 step {n + 2}", 0)
+    for _ in range(60):
+        sim.step()
+    _handover(sim, deck, f"c1 step {n + 64}", 200)
+
+
+def _digest(sim, i):
+    h = hashlib.sha256()
+    f = sim.fields(i)
+    for key in sorted(f):
+        h.update(key.encode())
+        h.update(np.ascontiguousarray(f[key]).tobytes())
+    h.update(sim.grains(i).tobytes())
+    return h.hexdigest(), sim.info(i)["tInd"]
+
+
+def test_batched_workload_64_instances_match_single_runs():
+    names = sorted(glob.glob(os.path.join(ROOT, "decks", "c2_batched", "*.in")))
+    assert len(names) == 64
+    decks = [amcafe_amd.read_deck(n) for n in names]
+    batch = amcafe_amd.Sim(decks)
+    batch.run()
+    want = [_digest(batch, i) for i in range(64)]
+    batch.close()
+    for i, d in enumerate(decks):
+        one = amcafe_amd.Sim(d)
+        one.run()
+        assert _digest(one, 0) == want[i], f"instance {i} ({os.path.basename(names[i])}) differs from the batch"
+        one.close()
