@@ -74,6 +74,9 @@ SIGNATURES = {
     "amcafe_heat_loss_f64": (_I, [_P, _P, _P, _I, _I, _P, _P]),
     "amcafe_heat_loss_grad_batch_f32": (_I, [_P, _I, _P, _P, _I, _I, _P, _P, _P]),
     "amcafe_heat_last_evaluations": (ctypes.c_longlong, []),
+    "amcafe_heat_loss_grad_batch_f32_noalloc": (_I, [_P, _I, _P, _P, _I, _I, _P, _P]),
+    "amcafe_heat_noalloc_bytes": (ctypes.c_longlong, [_P, _I]),
+    "amcafe_heat_noalloc_max_bytes": (ctypes.c_longlong, []),
     "amcafe_sim_set_timing": (_I, [_P, _I]),
     "amcafe_sim_timings": (_I, [_P, _P, _I]),
 }

@@ -109,6 +109,12 @@ int amcd_add_layer(amcd_batch* b, int i, int iz1, int ilaserLoc, int gbase, cons
 // and gradient (FP32), 2 loss (FP64); amcafe_heat_desc is declared in include/amcafe.h
 struct amcafe_heat_desc_s;
 unsigned long long amcd_heat_last_evals(void);
+// the FP32 batch loss and gradient with every device buffer in a static device array
+// (no allocation): 0 ok, -1 device error, -2 the call needs more than that array holds
+int amcd_heat_noalloc(const struct amcafe_heat_desc_s* d, int nb, const double* theta, const double* target,
+                      int step0, int nsteps, double* loss, double* grad, char* err, int errlen);
+size_t amcd_heat_noalloc_bytes(const struct amcafe_heat_desc_s* d, int nb);
+size_t amcd_heat_noalloc_max(void);
 int amcd_heat(const struct amcafe_heat_desc_s* d, int nb, const double* theta, const double* target, int step0,
               int nsteps, int mode, double* loss, double* grad, float* ms, char* err, int errlen);
 // EASM temperature (S/TempField.C:79-137) instead of the analytic one: beam power Q,

@@ -641,6 +641,22 @@ extern "C" int amcafe_heat_loss_f64(const amcafe_heat_desc* d, const double* the
                                     int nsteps, double* loss, float* ms) {
   return heat(d, 1, theta, target, step0, nsteps, 2, loss, nullptr, ms);
 }
+extern "C" int amcafe_heat_loss_grad_batch_f32_noalloc(const amcafe_heat_desc* d, int n, const double* theta,
+                                                       const double* target, int step0, int nsteps, double* loss,
+                                                       double* grad) {
+  if (!heat_args_ok(d, n, theta, target, step0, nsteps, loss) || !grad)
+    return fail(AMCAFE_ERR_ARG, "bad heat-field arguments");
+  char err[256] = {0};
+  const int rc = amcd_heat_noalloc(d, n, theta, target, step0, nsteps, loss, grad, err, sizeof err);
+  if (rc == -2) return fail(AMCAFE_ERR_ARG, err);
+  if (rc) return fail(AMCAFE_ERR_DEVICE, err);
+  return AMCAFE_OK;
+}
+extern "C" long long amcafe_heat_noalloc_bytes(const amcafe_heat_desc* d, int n) {
+  if (!d || n <= 0 || d->nX[0] <= 0 || d->nX[1] <= 0 || d->nX[2] <= 0) return -1;
+  return (long long)amcd_heat_noalloc_bytes(d, n);
+}
+extern "C" long long amcafe_heat_noalloc_max_bytes(void) { return (long long)amcd_heat_noalloc_max(); }
 extern "C" long long amcafe_heat_last_evaluations(void) { return (long long)amcd_heat_last_evals(); }
 extern "C" int amcafe_heat_loss_grad_batch_f32(const amcafe_heat_desc* d, int n, const double* theta,
                                                const double* target, int step0, int nsteps, double* loss, double* grad,

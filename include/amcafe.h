@@ -202,6 +202,20 @@ int amcafe_heat_loss_grad_batch_f32(const amcafe_heat_desc* d, int n, const doub
                                     int step0, int nsteps, double* loss, double* grad, float* ms);
 /* voxel-steps (per instance) the last heat call of this thread evaluated, after pruning */
 long long amcafe_heat_last_evaluations(void);
+/* amcafe_heat_loss_grad_batch_f32 without any device allocation, for callers that cannot
+ * allocate: code under HIP graph capture, serving loops, harnesses that forbid hipMalloc.  No
+ * counterpart in the reference.  Same kernels and the same bits as the batch entry point; no
+ * timing.  Its device buffers (parameters, block sums) live in one static device array of
+ * amcafe_heat_noalloc_max_bytes() (8 MiB) per GPU, held until the process ends: a call needing
+ * more (amcafe_heat_noalloc_bytes: about 20 bytes per 256 voxels and instance, so 4.2 M voxels
+ * for up to 25 instances) fails with AMCAFE_ERR_ARG before touching the device.  Every call on
+ * a GPU shares that GPU's array: calls that overlap on one GPU (two host threads, or two
+ * streams) race and return garbage. */
+int amcafe_heat_loss_grad_batch_f32_noalloc(const amcafe_heat_desc* d, int n, const double* theta,
+                                            const double* target, int step0, int nsteps, double* loss,
+                                            double* grad);
+long long amcafe_heat_noalloc_bytes(const amcafe_heat_desc* d, int n);
+long long amcafe_heat_noalloc_max_bytes(void);
 
 #ifdef __cplusplus
 }
