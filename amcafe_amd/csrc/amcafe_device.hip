@@ -2347,14 +2347,29 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             r = rpair;
             key = scan * 32 + lane;
           }
-          // position in (D, ruqs, key) order among the liquid neighbours: a uniform loop
-          // over the ballot's set bits, branch-free inside
+          // position in (D, ruqs, key) order among the liquid neighbours (cand_less): a
+          // uniform loop over the ballot's set bits that broadcasts only D counts the
+          // neighbours strictly before by D (NaN last) and marks those with an equal D;
+          // a second loop over the lanes in some tie orders those by (ruqs, key)
           int rank = 0;
+          unsigned long long tie = 0;
+          const bool nan_me = D != D;
           for (unsigned long long m = bal; m != 0; m &= m - 1) {
             const int j = (int)__builtin_ctzll(m);
-            const double Dj = lane_f64(D, j), rj = lane_f64(r, j);
+            const double Dj = lane_f64(D, j);
+            const bool nan_j = Dj != Dj;
+            const bool before = (nan_me & !nan_j) | (!nan_me & !nan_j & (Dj < D));
+            const bool same = (j != lane) & ((nan_me & nan_j) | (!nan_me & !nan_j & (Dj == D)));
+            rank += before ? 1 : 0;
+            tie |= (unsigned long long)same << j;
+          }
+          for (unsigned long long m = __ballot(liq & (tie != 0)); m != 0; m &= m - 1) {
+            const int j = (int)__builtin_ctzll(m);
+            const double rj = lane_f64(r, j);
             const int kj = __builtin_amdgcn_readlane(key, j);
-            rank += ((j != lane) & cand_less(Dj, rj, kj, D, r, key)) ? 1 : 0;
+            const bool rne = rj != r;
+            const bool before = (rne & (rj < r)) | (!rne & (kj < key));
+            rank += (((tie >> j) & 1ull) != 0 && before) ? 1 : 0;
           }
           if (liq) {
             d.cD[cb + rank] = D;

@@ -34,7 +34,9 @@ extern "C" {
 #define AMCAFE_ERR_ARG (-1)     /* bad argument or call order */
 #define AMCAFE_ERR_DEVICE (-2)  /* HIP runtime failure or no device */
 #define AMCAFE_ERR_DECK (-3)    /* deck could not be parsed / is incomplete */
-#define AMCAFE_ERR_STATE (-4)   /* state the path cannot handle (see message) */
+#define AMCAFE_ERR_STATE (-4)   /* state the path cannot handle (see message); a domain or
+                                   simulation whose step failed holds a partial step: upload its
+                                   state again (seam level) or recreate it before stepping on */
 #define AMCAFE_ERR_IO (-5)      /* output file could not be written */
 
 const char* amcafe_last_error(void);

@@ -106,7 +106,7 @@ def test_c1_handover_after_the_first_layer_change():
     _handover(sim, deck, f"c1 step {n + 2}", 0)
     for _ in range(60):
         sim.step()
-    _handover(sim, deck, f"c1 step {n + 64}", 200)
+    _handover(sim, deck, f"c1 step {n + 64}", 50)
 
 
 def _digest(sim, i):

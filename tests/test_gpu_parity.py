@@ -159,50 +159,43 @@ def seam_run(deck_name, max_steps, upload_after_temperature=False):
     return events
 
 
-def test_c4_production_lockstep_from_mid_run_state():
-    """The production run of the calibration-and-production form at full size (1024x512x128,
-    5000-grain base plate, 67 deg layer rotation): the GPU's state after 120 steps, taken over
-    by the correctly rounded oracle; the next two steps must agree field for field."""
-    deck = read("c4_production.in")
+def _mid_run_handover(deck_name, warm, label, min_events):
+    """The GPU's state after `warm` steps handed to both oracles (correctly rounded and glibc);
+    the next two steps field for field: every field bit-exact against the correctly rounded
+    oracle, grain ids and states bit-exact and octahedra within 1e-12 relative against glibc's."""
+    deck = read(deck_name)
     sim = amcafe_amd.Sim(deck)
-    for _ in range(120):
+    for _ in range(warm):
         sim.step()
     f = sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID"))
-    o = Oracle(deck, crmath=True, bare=True)
-    o.load_state(f, sim.grains(0), sim.info(0))
+    cr, gl = Oracle(deck, crmath=True, bare=True), Oracle(deck, crmath=False, bare=True)
+    for o in (cr, gl):
+        o.load_state(f, sim.grains(0), sim.info(0))
     for n in range(2):
         sim.step()
-        o.temperature()
-        o.update_voxels()
-        o.finish_step()
-        g, of = sim.fields(0), o.fields()
+        for o in (cr, gl):
+            o.temperature()
+            o.update_voxels()
+            o.finish_step()
+        g, of, lg = sim.fields(0), cr.fields(), gl.fields()
         for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
-            assert _same(g[k], of[k]), f"c4 step {121 + n}: " + _diff(k, g[k], of[k])
-        s, so = sim.stats(0), o.stats()
-        print(f"c4 step {121 + n}: {s['n_events']} events, {s['n_active']} interface cells")
-        assert s["n_events"] == so["n_events"] > 0
+            assert _same(g[k], of[k]), f"{label} step {warm + 1 + n}: " + _diff(k, g[k], of[k])
+        for k in ("gID", "vState"):
+            assert np.array_equal(g[k], lg[k]), f"{label} step {warm + 1 + n} (glibc oracle): " + _diff(k, g[k], lg[k])
+        np.testing.assert_allclose(g["extents"], lg["extents"], rtol=1e-12, atol=1e-21)
+        s, so = sim.stats(0), cr.stats()
+        print(f"{label} step {warm + 1 + n}: {s['n_events']} events, {s['n_active']} interface cells")
+        assert s["n_events"] == so["n_events"] == gl.stats()["n_events"] >= min_events
         assert s["n_active"] == so["n_active"]
-    assert np.array_equal(sim.grains(0), o.grains())
+    assert np.array_equal(sim.grains(0), cr.grains())
+
+
+def test_c4_production_lockstep_from_mid_run_state():
+    """The production run of the calibration-and-production form at full size (1024x512x128,
+    5000-grain base plate, 67 deg layer rotation) from the GPU's state after 120 steps."""
+    _mid_run_handover("c4_production.in", 120, "c4", 1)
 
 
 def test_c1_full_size_lockstep_from_mid_run_state():
-    """Workload c1 at full size (512x512x128): the GPU's state after 150 steps, taken
-    over by the correctly rounded oracle; the next steps must agree field for field."""
-    deck = read("c1_raster.in")
-    sim = amcafe_amd.Sim(deck)
-    for _ in range(150):
-        sim.step()
-    f = sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID"))
-    o = Oracle(deck, crmath=True, bare=True)
-    o.load_state(f, sim.grains(0), sim.info(0))
-    for n in range(2):
-        sim.step()
-        o.temperature()
-        o.update_voxels()
-        o.finish_step()
-        g, of = sim.fields(0), o.fields()
-        for k in ("gID", "vState", "BuildjID", "TempCurr", "extents", "centroidOct"):
-            assert _same(g[k], of[k]), f"c1 step {151 + n}: " + _diff(k, g[k], of[k])
-        s, so = sim.stats(0), o.stats()
-        assert s["n_events"] == so["n_events"] > 100
-    assert np.array_equal(sim.grains(0), o.grains())
+    """Workload c1 at full size (512x512x128) from the GPU's state after 150 steps."""
+    _mid_run_handover("c1_raster.in", 150, "c1", 100)
