@@ -21,7 +21,7 @@ class Stats(ctypes.Structure):
                 ("n_captures", ctypes.c_int), ("n_nucleations", ctypes.c_int), ("n_growing", ctypes.c_int),
                 ("nGrain", ctypes.c_int), ("draws", ctypes.c_longlong), ("tinc", ctypes.c_double),
                 ("slot_visits", ctypes.c_longlong), ("cand_evals", ctypes.c_longlong),
-                ("interface_events", ctypes.c_longlong)]
+                ("interface_events", ctypes.c_longlong), ("loop_cycles", ctypes.c_longlong)]
 
 
 class Info(ctypes.Structure):

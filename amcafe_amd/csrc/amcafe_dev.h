@@ -65,6 +65,7 @@ typedef struct {
   long long slot_visits;  // growing cells with live candidates, summed over the loop's scans
   long long cand_evals;   // capture times evaluated, summed over the loop's scans
   long long interface_events;  // growing-list length summed over the loop's events
+  long long loop_cycles;       // shader-clock cycles of the capture loop, summed over its launches
 } amcd_out;
 
 typedef struct amcd_batch amcd_batch;
@@ -82,6 +83,8 @@ int amcd_download(amcd_batch* b, int i, int* gid, int* vstate, double* ext, doub
 int amcd_set_rot(amcd_batch* b, int i, const double* rot9, int ngrain);
 int amcd_get_rot(amcd_batch* b, int i, double* rot9, int ngrain);
 int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n);
+// every instance's pool in one staged copy (rot9[i] NULL or n[i] = 0: instance i unchanged)
+int amcd_set_pools(amcd_batch* b, const double* const* rot9, const int* n);
 
 // the time step: all instances of the batch advance together.
 // stage 1: temperature only (AnalyticTempCurr seam)
@@ -133,7 +136,7 @@ void amcd_set_timing(amcd_batch* b, int on);
 int amcd_last_timings(amcd_batch* b, double* ms, int n);
 const char* amcd_last_error(void);
 // capture-loop phase timers of a -DAMCD_PHASE_PROFILE build (zeros otherwise); resets them
-int amcd_phase_profile(unsigned long long* out16);
+int amcd_phase_profile(unsigned long long* out32);
 
 // known-answer helpers: libstdc++ uniform_real_distribution<double> over
 // minstd_rand0(seed), sequentially and through the device's jump-ahead path

@@ -44,7 +44,7 @@
 // Phase timers of the capture loop (core-clock cycles summed over events), compiled in
 // only with -DAMCD_PHASE_PROFILE; read with amcd_phase_profile().
 #ifdef AMCD_PHASE_PROFILE
-__device__ unsigned long long g_phase[16];
+__device__ unsigned long long g_phase[32];
 #define PHASE_INIT(v) unsigned long long v = clock64()
 #define PHASE(v, slot, pred)                                \
   do {                                                      \
@@ -52,18 +52,64 @@ __device__ unsigned long long g_phase[16];
     if (pred) atomicAdd(&g_phase[slot], t_ - v);            \
     v = t_;                                                 \
   } while (0)
+// barrier arrival of wave w (lane 0) with a flag word of what the wave did this event
+#define PHASE_ARRIVE(sh, par, w, flag)         \
+  do {                                         \
+    int wf_ = 0;                               \
+    for (int b_ = 0; b_ < 6; ++b_)             \
+      if (__ballot(((flag) >> b_) & 1)) wf_ |= 1 << b_; \
+    if ((threadIdx.x & 63) == 0) {             \
+      (sh).arr[par][w] = clock64();            \
+      (sh).aflag[par][w] = wf_;                \
+    }                                          \
+  } while (0)
+// after the barrier: which wave arrived last, by how much, and what it had done
+#define PHASE_LAST(sh, par, NW)                                              \
+  do {                                                                       \
+    if (threadIdx.x == 0) {                                                  \
+      int wl = 0;                                                            \
+      unsigned long long mx = 0, mn = ~0ull, m2 = 0;                         \
+      for (int q = 0; q < (NW); ++q) {                                       \
+        const unsigned long long a = (sh).arr[par][q];                       \
+        if (a > mx) { m2 = mx; mx = a; wl = q; } else if (a > m2) m2 = a;    \
+        if (a < mn) mn = a;                                                  \
+      }                                                                      \
+      atomicAdd(&g_phase[16 + wl], 1ull);                                    \
+      atomicAdd(&g_phase[24], mx - mn);                                      \
+      atomicAdd(&g_phase[25], mx - m2);                                      \
+      const int fl = (sh).aflag[par][wl];                                    \
+      for (int b = 0; b < 6; ++b)                                            \
+        if ((fl >> b) & 1) atomicAdd(&g_phase[26 + b], 1ull);                \
+    }                                                                        \
+  } while (0)
 #else
 #define PHASE_INIT(v)
 #define PHASE(v, slot, pred)
+#define PHASE_ARRIVE(sh, par, w, flag)
+#define PHASE_LAST(sh, par, NW)
 #endif
 
 // A pointer known to address global memory.  The capture loop reads its array pointers
 // from a descriptor copied to LDS, so the compiler cannot tell their address space and
 // would emit flat accesses, which also count against the LDS wait counter: every LDS
-// wait would then wait for the outstanding memory loads as well.
+// wait would then wait for the outstanding memory loads, and for the stores to be
+// acknowledged, as well.  A flat store may also alias the descriptor in LDS, so each one
+// makes the compiler read the next array pointer from LDS again.
 template <class T>
 __device__ __forceinline__ __attribute__((address_space(1))) T* gmem(T* p) {
   return (__attribute__((address_space(1))) T*)p;
+}
+// 8-byte vector records (int2, float2) read and written through a global pointer as one
+// 64-bit access (HIP's vector types have no assignment in that address space)
+template <class V>
+__device__ __forceinline__ V gload8(const V* p) {
+  static_assert(sizeof(V) == 8, "8-byte records only");
+  return __builtin_bit_cast(V, *gmem((const unsigned long long*)p));
+}
+template <class V>
+__device__ __forceinline__ void gstore8(V* p, V v) {
+  static_assert(sizeof(V) == 8, "8-byte records only");
+  *gmem((unsigned long long*)p) = __builtin_bit_cast(unsigned long long, v);
 }
 
 // lane l's value of v (l must be the same in every lane)
@@ -1467,6 +1513,10 @@ struct EngineShared {
   int nslots;
   int r_i1;         // resume: pending nucleation target
   int fin;          // exit code of the loop
+#ifdef AMCD_PHASE_PROFILE
+  unsigned long long arr[2][16];
+  int aflag[2][16];
+#endif
 };
 // The decision every wave takes after the barrier (S/VoxelsCA.C:476-485).
 struct Decision {
@@ -1710,6 +1760,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   x.i1 = sh.r_i1;
   x.dt = 0.0;
   bool apply_first = go == 2;
+  int aflag = 0;  // phase profile only: what this lane did in the event
+  (void)aflag;
   int ks = -1;  // the cell whose list block pb is loaded for its head refresh (-1: none)
   SlotBlock pb = slot_block_none();
   PHASE_INIT(ot);
@@ -1766,6 +1818,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
             // times are non-decreasing along the list: a tie needs the next entry's
             // time to equal t1; then walk the tied block
             if (ok & (tn == t1)) {
+              aflag |= 2;
               const int n = gmem(d.gN)[s];
               for (int q = pos + 1; q < n; ++q) {
                 const int tq = gmem(d.cTgt)[c - pos + q];
@@ -1789,6 +1842,8 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       b = wave_best(b);
       if (lane == 0) sh.sb[par][w] = b;
       PHASE(ot, 7, tid == 64);
+      PHASE_ARRIVE(sh, par, w, aflag);
+      aflag = 0;
       __syncthreads();
       PHASE(ot, 8, tid == 64);
       // ---- DECIDE
@@ -1831,6 +1886,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
     // the cell created by the previous event (its best was evaluated by the engine)
     const NewCell& nc = sh.mb[par];
     if (nc.valid && nc.slot % NO == mo) {
+      aflag |= 4;
       const int kk = nc.slot / NO;
       double e = nc.E;
       if (!x.nuc) {
@@ -1868,6 +1924,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         const int s = mo + k * NO;
         if (tgt1[k] == i1) {
           const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
+          aflag |= (HC && k == ks) ? 8 : 1;
           const SlotHead h = (HC && k == ks) ? slot_head_pre(d, dead, s, pos + 1, i1, pb)
                                              : slot_head(d, dead, s, pos + 1, i1);
           tgt1[k] = h.tgt1;
@@ -1882,6 +1939,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
           lo[k] = hi[k] = fInf;
           if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
         } else {
+          aflag |= 16;
           tau_bounds(HC ? hc[s] : gmem(d.cD)[c1[k]], E[k], V[k], T, lo[k], hi[k]);
         }
       }
@@ -1948,9 +2006,9 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
   for (int s = me; s < ns; s += NO) {
     const SlotHead h = slot_head(d, dead, s, 0, -1);
     float lo = fInf, hi = fInf;
-    if (h.tgt1 >= 0) tau_bounds(h.D1, d.gExt[s], d.gVhat[s], T, lo, hi);
-    d.gHd[s] = make_int2(h.c1, h.tgt1);
-    d.gLH[s] = make_float2(lo, hi);
+    if (h.tgt1 >= 0) tau_bounds(h.D1, gmem(d.gExt)[s], gmem(d.gVhat)[s], T, lo, hi);
+    gstore8(d.gHd + s, make_int2(h.c1, h.tgt1));
+    gstore8(d.gLH + s, make_float2(lo, hi));
     note(s, make_float2(lo, hi));
   }
   int skip = -1;
@@ -1972,15 +2030,15 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
       const float u = wave_min(hmin);
       Best b = best_none();
       auto evaluate = [&](int s) {  // exact capture time of slot s's head (S/VoxelsCA.C:420-440)
-        const int2 hd = d.gHd[s];
+        const int2 hd = gload8(d.gHd + s);
         const int c = hd.x;
         const int pos = c - s * AMCD_NB;
-        const double e = d.gExt[s], v = d.gVhat[s];
-        const double D1 = d.cD[c];
-        const double Dn = (pos + 1 < AMCD_NB) ? d.cD[c + 1] : __builtin_inf();
-        const double rr = d.cR[c];
-        const int kb = d.gScan[s] * 32;
-        const int kk = d.cK[c];
+        const double e = gmem(d.gExt)[s], v = gmem(d.gVhat)[s];
+        const double D1 = gmem(d.cD)[c];
+        const double Dn = (pos + 1 < AMCD_NB) ? gmem(d.cD)[c + 1] : __builtin_inf();
+        const double rr = gmem(d.cR)[c];
+        const int kb = gmem(d.gScan)[s] * 32;
+        const int kk = gmem(d.cK)[c];
         visits++;
         evals++;
         const double t1 = (D1 - e) / v;
@@ -1992,17 +2050,17 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
           y.cand = c;
           y.tgt = hd.y;
           if ((Dn - e) / v == t1) {
-            const int n = d.gN[s];
+            const int n = gmem(d.gN)[s];
             for (int q = pos + 1; q < n; ++q) {
-              const int tq = d.cTgt[c - pos + q];
+              const int tq = gmem(d.cTgt)[c - pos + q];
               if (!live_target(dead, tq, skip)) continue;
               evals++;
-              const double tt = (d.cD[c - pos + q] - e) / v;
+              const double tt = (gmem(d.cD)[c - pos + q] - e) / v;
               if (!(tt == t1)) break;
               Best z;
               z.t = tt;
-              z.r = d.cR[c - pos + q];
-              z.key = kb + d.cK[c - pos + q];
+              z.r = gmem(d.cR)[c - pos + q];
+              z.key = kb + gmem(d.cK)[c - pos + q];
               z.cand = c - pos + q;
               z.tgt = tq;
               if (better(z, y)) y = z;
@@ -2017,7 +2075,7 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
 #pragma unroll
           for (int q = 0; q < MB; ++q) {
             const int s = s0 + q * NO;
-            lob[q] = s < ns ? d.gLH[s].x : fInf;
+            lob[q] = s < ns ? gload8(d.gLH + s).x : fInf;
           }
 #pragma unroll
           for (int q = 0; q < MB; ++q)
@@ -2042,10 +2100,10 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
     // below grows it like every other cell and computes its bounds
     const NewCell& nc = sh.mb[par];
     if (nc.valid && nc.slot % NO == me) {
-      d.gExt[nc.slot] = nc.E;
-      d.gVhat[nc.slot] = nc.V;
-      d.gHd[nc.slot] = make_int2(nc.c1, nc.tgt1);
-      d.gLH[nc.slot] = make_float2(-fInf, fInf);
+      gmem(d.gExt)[nc.slot] = nc.E;
+      gmem(d.gVhat)[nc.slot] = nc.V;
+      gstore8(d.gHd + nc.slot, make_int2(nc.c1, nc.tgt1));
+      gstore8(d.gLH + nc.slot, make_float2(-fInf, fInf));
     }
     if (nc.valid) ns = max(ns, nc.slot + 1);
     T = T + Dt;
@@ -2062,10 +2120,10 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
       for (int q = 0; q < MB; ++q) {
         const int s = s0 + q * NO;
         const bool in = s < ns;
-        hdb[q] = in ? d.gHd[s] : make_int2(0, -1);
-        lhb[q] = in ? d.gLH[s] : make_float2(fInf, fInf);
-        eb[q] = in ? d.gExt[s] : 0.0;
-        vb[q] = in ? d.gVhat[s] : 0.0;
+        hdb[q] = in ? gload8(d.gHd + s) : make_int2(0, -1);
+        lhb[q] = in ? gload8(d.gLH + s) : make_float2(fInf, fInf);
+        eb[q] = in ? gmem(d.gExt)[s] : 0.0;
+        vb[q] = in ? gmem(d.gVhat)[s] : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < MB; ++q) {
@@ -2083,18 +2141,18 @@ __device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst&
           } else {
             e = g;
           }
-          d.gExt[s] = e;
+          gmem(d.gExt)[s] = e;
         }
         if (hd.y == i1) {
           const SlotHead h = slot_head(d, dead, s, hd.x - s * AMCD_NB + 1, i1);
           hd = make_int2(h.c1, h.tgt1);
           lh = make_float2(fInf, fInf);
           if (h.tgt1 >= 0) tau_bounds(h.D1, e, v, T, lh.x, lh.y);
-          d.gHd[s] = hd;
-          d.gLH[s] = lh;
+          gstore8(d.gHd + s, hd);
+          gstore8(d.gLH + s, lh);
         } else if (lh.x == -fInf || all) {
-          tau_bounds(d.cD[hd.x], e, v, T, lh.x, lh.y);
-          d.gLH[s] = lh;
+          tau_bounds(gmem(d.cD)[hd.x], e, v, T, lh.x, lh.y);
+          gstore8(d.gLH + s, lh);
         }
         note(s, lh);
       }
@@ -2196,6 +2254,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
       cb = mod_m((unsigned long long)cb * c128);
     }
     int par = 0, fin = 0;
+    const unsigned long long c_start = clock64();  // the loop's own duration (stats)
     Decision x;
     x.stop = 0;
     x.nuc = 1;
@@ -2206,12 +2265,17 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     bool apply_first = go == 2;
     EnginePrefetch pf;
     bool have_pf = false;  // the event's inputs were prefetched with its decision
+    int eflag = 0;  // phase profile only: what the engine did in the event
+    (void)eflag;
     PHASE_INIT(et);
     while (go > 0) {
       if (!apply_first) {
         PHASE(et, 0, tid == 0);
+        PHASE_ARRIVE(sh, par, 0, eflag);
+        eflag = 0;
         __syncthreads();
         PHASE(et, 1, tid == 0);
+        PHASE_LAST(sh, par, NW);
         {
           const Best me = lane_entry<NW>(sh, par);
           pf = engine_prefetch<NW>(d, sh, me, par, ns_pre);
@@ -2241,8 +2305,8 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
       if (lane == 0) nc.valid = 0;
       if (x.nuc && (pool_used >= st.n_pool || pool_used >= d.poolcap || nGrain >= d.gcap)) {
         if (lane == 0) {
-          ls->pending = 1;  // the host refills the pool and resumes
-          ls->pend_tgt = i1;
+          *gmem(&ls->pending) = 1;  // the host refills the pool and resumes
+          *gmem(&ls->pend_tgt) = i1;
         }
         susp = -1;
       } else {
@@ -2271,55 +2335,63 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
                                 lane_f64(pf.rc.w, wq));
           }
         } else {
-          tn = (lane < AMCD_NB) ? d.aNbr[(size_t)i1 * AMCD_NB + lane] : -1;
-          rpair = (lane < AMCD_NB) ? d.aRN[(size_t)i1 * AMCD_NB + lane] : 0.0;
-          cold = d.aCold[i1];
-          T1 = d.aT[i1];
-          v1 = cand >= 0 ? d.cV[cand] : d.aVox[i1];  // (no entry: a resumed nucleation)
+          tn = (lane < AMCD_NB) ? gmem(d.aNbr)[(size_t)i1 * AMCD_NB + lane] : -1;
+          rpair = (lane < AMCD_NB) ? gmem(d.aRN)[(size_t)i1 * AMCD_NB + lane] : 0.0;
+          cold = gmem(d.aCold)[i1];
+          T1 = gmem(d.aT)[i1];
+          v1 = cand >= 0 ? gmem(d.cV)[cand] : gmem(d.aVox)[i1];  // (no entry: a resumed nucleation)
           if (!x.nuc) {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) R[q] = d.gR[9 * (size_t)sw + q];
-            G = d.gG[sw];
-            if (pre) rc = d.cRC[cand];
+            for (int q = 0; q < 9; ++q) R[q] = gmem(d.gR)[9 * (size_t)sw + q];
+            G = gmem(d.gG)[sw];
+            if (pre) {
+              const auto q = gmem((double*)(d.cRC + cand));
+              rc = make_double4(q[0], q[1], q[2], q[3]);
+            }
           }
         }
+        PHASE(et, 9, tid == 0);
         if (!x.nuc && !pre) {  // a cell made in this launch: re-centred below
 #pragma unroll
-          for (int q = 0; q < 3; ++q) c0[q] = d.gC[3 * (size_t)sw + q];
-          fcap = d.cF[cand];
-          vjs = d.gV[sw];
+          for (int q = 0; q < 3; ++q) c0[q] = gmem(d.gC)[3 * (size_t)sw + q];
+          fcap = gmem(d.cF)[cand];
+          vjs = gmem(d.gV)[sw];
         }
         if (x.nuc) {  // S/VoxelsCA.C:486-506
-          const double* P = d.pool + 9 * (size_t)pool_used;
+          eflag |= 8;
+          const auto P = gmem(d.pool) + 9 * (size_t)pool_used;
           for (int q = 0; q < 9; ++q) R[q] = P[q];
           G = nGrain + 1;
           centre(d, v1, cen);
           E0 = 0.0;
-          if (lane < 9) d.rot[9 * (size_t)(G - 1) + lane] = P[lane];
+          if (lane < 9) gmem(d.rot)[9 * (size_t)(G - 1) + lane] = P[lane];
           nGrain = G;
           pool_used++;
           nucs++;
-          if (lane == 0) ls->pending = 0;
+          if (lane == 0) *gmem(&ls->pending) = 0;
         } else {      // S/VoxelsCA.C:507-605
           if (pre) {
             cen[0] = rc.x;
             cen[1] = rc.y;
             cen[2] = rc.z;
             E0 = rc.w;
+            eflag |= 1;
           } else {
+            eflag |= 2;
             recentre(d, vjs, v1, c0, R, fcap, cen, &E0);
           }
           captures++;
         }
+        PHASE(et, 10, tid == 0);
         const int scan = vSind2;
         if (lane == 0) {
-          d.aG[i1] = G;
-          d.aCx[i1] = cen[0];
-          d.aCy[i1] = cen[1];
-          d.aCz[i1] = cen[2];
-          d.aExt[i1] = E0;
-          d.aVS[i1] = 2;
-          d.aScan[i1] = scan;
+          gmem(d.aG)[i1] = G;
+          gmem(d.aCx)[i1] = cen[0];
+          gmem(d.aCy)[i1] = cen[1];
+          gmem(d.aCz)[i1] = cen[2];
+          gmem(d.aExt)[i1] = E0;
+          gmem(d.aVS)[i1] = 2;
+          gmem(d.aScan)[i1] = scan;
           dead[i1 >> 5] |= 1u << (i1 & 31);
         }
         vSind2++;
@@ -2329,6 +2401,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         const bool liq = live_target(dead, tn, i1);
         const unsigned long long bal = __ballot(liq);
         if (cold && bal) {
+          eflag |= 4;
           const int slot = nslots;
           const size_t cb = (size_t)slot * AMCD_NB;
           const double V0 = growth_velocity(p, T1);
@@ -2372,12 +2445,12 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             rank += (((tie >> j) & 1ull) != 0 && before) ? 1 : 0;
           }
           if (liq) {
-            d.cD[cb + rank] = D;
-            d.cF[cb + rank] = f;
-            d.cR[cb + rank] = r;
-            d.cTgt[cb + rank] = tn;
-            d.cV[cb + rank] = vt;
-            d.cK[cb + rank] = (uint8_t)lane;
+            gmem(d.cD)[cb + rank] = D;
+            gmem(d.cF)[cb + rank] = f;
+            gmem(d.cR)[cb + rank] = r;
+            gmem(d.cTgt)[cb + rank] = tn;
+            gmem(d.cV)[cb + rank] = vt;
+            gmem(d.cK)[cb + rank] = (uint8_t)lane;
             if (rank == 0) {
               nc.D1 = D;
               nc.r1 = r;
@@ -2408,17 +2481,17 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
 #pragma unroll
           for (int q = 0; q < 3; ++q)
             if (q + 9 == lane) rv = cen[q];
-          if (lane < 9) d.gR[9 * (size_t)slot + lane] = rv;
-          if (lane >= 9 && lane < 12) d.gC[3 * (size_t)slot + lane - 9] = rv;
+          if (lane < 9) gmem(d.gR)[9 * (size_t)slot + lane] = rv;
+          if (lane >= 9 && lane < 12) gmem(d.gC)[3 * (size_t)slot + lane - 9] = rv;
           if (lane == 0) {
-            d.gV[slot] = v1;
-            d.gG[slot] = G;
-            d.gAct[slot] = i1;
-            d.gScan[slot] = scan;
-            d.gExt[slot] = E0;
-            d.gVhat[slot] = V0;
-            d.gN[slot] = (uint8_t)__popcll(bal);
-            d.gDead[slot] = 0;
+            gmem(d.gV)[slot] = v1;
+            gmem(d.gG)[slot] = G;
+            gmem(d.gAct)[slot] = i1;
+            gmem(d.gScan)[slot] = scan;
+            gmem(d.gExt)[slot] = E0;
+            gmem(d.gVhat)[slot] = V0;
+            gmem(d.gN)[slot] = (uint8_t)__popcll(bal);
+            gmem(d.gDead)[slot] = 0;
             nc.E = E0;
             nc.V = V0;
             nc.slot = slot;
@@ -2453,7 +2526,10 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     }
     visits += e_visits;
     evals += e_evals;
-    if (tid == 0) atomicAdd((unsigned long long*)&O[blockIdx.x].interface_events, (unsigned long long)e_units);
+    if (tid == 0) {
+      atomicAdd((unsigned long long*)&O[blockIdx.x].interface_events, (unsigned long long)e_units);
+      atomicAdd((unsigned long long*)&O[blockIdx.x].loop_cycles, clock64() - c_start);
+    }
   }
   __syncthreads();
   // ---- loop state and counters
@@ -2691,6 +2767,12 @@ struct amcd_batch {
   int tier = 1;  // capture-loop capacity tier (see launch_event_loop)
   int min_tier = 0;  // lowest tier a step starts in (AMCAFE_CAPTURE_TIER_MIN; tests use it to
                      // run small decks in the larger tiers; 5 = memory tier, bitmap in memory)
+  // nucleation pools of every instance in one device region (instance i's at pool_off[i]
+  // entries of 9 doubles), filled from one pinned staging copy per step
+  double* d_pool = nullptr;
+  double* h_pool = nullptr;  // pinned
+  std::vector<size_t> pool_off;
+  hipEvent_t pool_ev = nullptr;  // the last staging copy (the staging area is rewritten after it)
 };
 
 static thread_local std::string g_dev_err;
@@ -2891,6 +2973,9 @@ void amcd_destroy(amcd_batch* b) {  // best effort: errors while tearing down ar
     if (b->ev[q]) (void)hipEventDestroy(b->ev[q]);
   if (b->h_steps) (void)hipHostFree(b->h_steps);
   if (b->h_outs) (void)hipHostFree(b->h_outs);
+  if (b->h_pool) (void)hipHostFree(b->h_pool);
+  if (b->d_pool) (void)hipFree(b->d_pool);
+  if (b->pool_ev) (void)hipEventDestroy(b->pool_ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -2994,19 +3079,63 @@ int amcd_get_rot(amcd_batch* b, int i, double* rot9, int ngrain) {
   return 0;
 }
 
-int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n) {
-  Inst& d = b->h[i];
-  if (n > d.poolcap) {
-    double* np = nullptr;
-    int ncap = std::max(n, 64);
-    HIPT(hipMalloc(&np, sizeof(double) * 9 * (size_t)ncap));
-    b->allocs.push_back(np);
-    d.pool = np;
-    d.poolcap = ncap;
+// Every instance's pool lives in one device region behind one pinned staging area: the
+// pools of a step go to the device in one copy instead of one pageable copy per instance
+// (each of which the host waits for).  A pool larger than its instance's capacity
+// re-lays the region out for every instance (capacities at least double).
+int amcd_set_pools(amcd_batch* b, const double* const* rot9, const int* n) {
+  bool grow = b->d_pool == nullptr;
+  for (int i = 0; i < b->n; ++i) grow = grow || n[i] > b->h[i].poolcap;
+  if (grow) {
+    HIPT(hipStreamSynchronize(b->stream));  // no launch reads the old pools any more
+    std::vector<size_t> off(b->n + 1, 0);
+    for (int i = 0; i < b->n; ++i) {
+      Inst& d = b->h[i];
+      int cap = std::max(64, d.poolcap);
+      while (cap < n[i]) cap *= 2;
+      d.poolcap = cap;
+      off[i + 1] = off[i] + (size_t)cap;
+    }
+    double *dp = nullptr, *hp = nullptr;
+    HIPT(hipMalloc(&dp, sizeof(double) * 9 * off[b->n]));
+    if (hipHostMalloc((void**)&hp, sizeof(double) * 9 * off[b->n], 0) != hipSuccess) {
+      (void)hipFree(dp);
+      g_dev_err = "hipHostMalloc of the pool staging area failed";
+      return -1;
+    }
+    if (b->d_pool) HIPT(hipFree(b->d_pool));
+    if (b->h_pool) HIPT(hipHostFree(b->h_pool));
+    b->d_pool = dp;
+    b->h_pool = hp;
+    b->pool_off = off;
+    for (int i = 0; i < b->n; ++i) b->h[i].pool = dp + 9 * off[i];
+    if (!b->pool_ev) HIPT(hipEventCreateWithFlags(&b->pool_ev, hipEventDisableTiming));
     if (push_inst(b)) return -1;
+  } else {
+    HIPT(hipEventSynchronize(b->pool_ev));  // the previous copy has read the staging area
   }
-  if (n > 0) HIPT(hipMemcpyAsync(d.pool, rot9, sizeof(double) * 9 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+  size_t lo = (size_t)-1, hi = 0;
+  for (int i = 0; i < b->n; ++i) {
+    if (n[i] <= 0 || !rot9[i]) continue;
+    const size_t o = b->pool_off[i];
+    memcpy(b->h_pool + 9 * o, rot9[i], sizeof(double) * 9 * (size_t)n[i]);
+    lo = std::min(lo, o);
+    hi = std::max(hi, o + (size_t)n[i]);
+  }
+  if (hi > lo)
+    HIPT(hipMemcpyAsync(b->d_pool + 9 * lo, b->h_pool + 9 * lo, sizeof(double) * 9 * (hi - lo),
+                        hipMemcpyHostToDevice, b->stream));
+  HIPT(hipEventRecord(b->pool_ev, b->stream));
   return 0;
+}
+
+int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n) {
+  if (i < 0 || i >= b->n) return -1;
+  std::vector<const double*> r(b->n, nullptr);
+  std::vector<int> c(b->n, 0);
+  r[i] = rot9;
+  c[i] = n;
+  return amcd_set_pools(b, r.data(), c.data());
 }
 
 // grain-table capacity for the coming step: current grains + pool size
@@ -3192,14 +3321,14 @@ static void collect_timing(amcd_batch* b) {
 int amcd_read_outs(amcd_batch* b, amcd_out* outs) { return read_outs(b, outs); }
 
 // capture-loop phase timers (zeros unless built with -DAMCD_PHASE_PROFILE); reading resets them
-int amcd_phase_profile(unsigned long long* out16) {
+int amcd_phase_profile(unsigned long long* out32) {
 #ifdef AMCD_PHASE_PROFILE
   HIPT(hipDeviceSynchronize());
-  HIPT(hipMemcpyFromSymbol(out16, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * 16));
-  unsigned long long z[16] = {0};
+  HIPT(hipMemcpyFromSymbol(out32, HIP_SYMBOL(g_phase), sizeof(unsigned long long) * 32));
+  unsigned long long z[32] = {0};
   HIPT(hipMemcpyToSymbol(HIP_SYMBOL(g_phase), z, sizeof(z)));
 #else
-  for (int q = 0; q < 16; ++q) out16[q] = 0;
+  for (int q = 0; q < 32; ++q) out32[q] = 0;
 #endif
   return 0;
 }

@@ -89,11 +89,18 @@ struct Pool {
 // the nucleation pools of instances that used them up.
 // Pool size for an instance whose previous step nucleated `last` grains: twice that plus a
 // margin, so that a refill (another launch of the capture loop) is rare.
-static int pool_size(int last) { return kPoolChunk + 2 * (last > 0 ? last : 0); }
+// AMCAFE_POOL_CHUNK overrides the margin (tests set 1 so that steps refill their pools)
+static int pool_size(int last) {
+  const char* e = getenv("AMCAFE_POOL_CHUNK");
+  const int chunk = e && atoi(e) > 0 ? atoi(e) : kPoolChunk;
+  return chunk + 2 * (last > 0 ? last : 0);
+}
 
 static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<amcd_out>& out,
                       std::vector<Pool>& pools, const Orientation& sa, bool fused, const std::vector<int>& last_nucs) {
   const int n = (int)st.size();
+  std::vector<const double*> prot(n, nullptr);
+  std::vector<int> pn(n, 0);
   for (int i = 0; i < n; ++i) {
     if (st[i].skip) continue;
     const int np = pool_size(last_nucs[i]);
@@ -101,12 +108,16 @@ static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<a
     pools[i].build(sa, st[i].tInd, st[i].nGrain, np);
     st[i].n_pool = np;
     if (amcd_reserve_grains(dev, i, st[i].nGrain + 4 * np + 64)) return dev_fail();
-    if (amcd_set_pool(dev, i, pools[i].rot.data(), np)) return dev_fail();
+    prot[i] = pools[i].rot.data();
+    pn[i] = np;
   }
+  if (amcd_set_pools(dev, prot.data(), pn.data())) return dev_fail();
   int rc = fused ? amcd_step_fused(dev, st.data(), out.data()) : amcd_update_voxels(dev, st.data(), out.data());
   if (rc) return dev_fail();
   for (int round = 0; round < 1 << 20; ++round) {
     bool pend = false;
+    std::fill(prot.begin(), prot.end(), nullptr);
+    std::fill(pn.begin(), pn.end(), 0);
     for (int i = 0; i < n; ++i) {
       if (out[i].status < 0 && !st[i].skip) {
         if (out[i].overflow) return fail(AMCAFE_ERR_DEVICE, "active list capacity could not be grown");
@@ -120,9 +131,11 @@ static int run_update(amcd_batch* dev, std::vector<amcd_step>& st, std::vector<a
       pools[i].build(sa, st[i].tInd, st[i].nGrain, nn);
       st[i].n_pool = nn;
       if (amcd_reserve_grains(dev, i, st[i].nGrain + nn + 64)) return dev_fail();
-      if (amcd_set_pool(dev, i, pools[i].rot.data(), nn)) return dev_fail();
+      prot[i] = pools[i].rot.data();
+      pn[i] = nn;
     }
     if (!pend) break;
+    if (amcd_set_pools(dev, prot.data(), pn.data())) return dev_fail();
     if (amcd_resume(dev, st.data(), out.data())) return dev_fail();
   }
   for (int i = 0; i < n; ++i)
@@ -145,6 +158,7 @@ static void to_stats(const amcd_out& o, amcafe_stats* s) {
   s->slot_visits = o.slot_visits;
   s->cand_evals = o.cand_evals;
   s->interface_events = o.interface_events;
+  s->loop_cycles = o.loop_cycles;
 }
 
 // ====================================================================================

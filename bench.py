@@ -279,7 +279,8 @@ def main():
     value = nvox * steps / wall
     sim.set_timing(True)
     stage_ms = np.zeros(6)
-    tot = dict(slot_visits=0, cand_evals=0, interface_events=0, n_events=0, n_active=0, n_growing=0)
+    tot = dict(slot_visits=0, cand_evals=0, interface_events=0, n_events=0, n_active=0, n_growing=0,
+               loop_cycles=0)
     tsteps = 0
     for _ in range(args.steps):
         if running == 0:
@@ -318,6 +319,9 @@ def main():
                 "algorithmic_bytes_per_launch": round(per_launch, 1), "window": window + " (HIP events)"}
     ev = tot["n_events"] / tsteps
     cap_us_per_event = (stage_ms[4] / tsteps) * 1e3 / ev if ev > 0 else None
+    # the loop's own clock (shader cycles between its first decision and its exit, engine wave)
+    cyc_per_event = tot["loop_cycles"] / tot["n_events"] if tot["n_events"] > 0 else None
+    loop_clock_ghz = tot["loop_cycles"] / (stage_ms[4] * 1e6) if stage_ms[4] > 0 else None
 
     cpu = None
     if snap is not None:
@@ -357,6 +361,8 @@ def main():
                    "stage_steps": tsteps,
                    "events_per_step": round(ev, 1),
                    "capture_us_per_event": None if cap_us_per_event is None else round(cap_us_per_event, 3),
+                   "capture_cycles_per_event": None if cyc_per_event is None else round(cyc_per_event, 1),
+                   "capture_loop_clock_GHz": None if loop_clock_ghz is None else round(loop_clock_ghz, 3),
                    "active_cells_per_step": round(tot["n_active"] / tsteps, 1),
                    "growing_cells_per_step": round(tot["n_growing"] / tsteps, 1)},
         "roofline": roof, "cpu_baseline": cpu, "full_run": whole, "batched": batched, "adjoint": adjoint,

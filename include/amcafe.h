@@ -81,6 +81,8 @@ typedef struct {
   long long cand_evals;  /* capture times evaluated, summed over the loop's scans */
   long long interface_events; /* growing cells summed over the capture loop's events (the
                                  cells the reference scans per event, S/VoxelsCA.C:386-400) */
+  long long loop_cycles;      /* the capture loop's duration in shader-clock cycles (read by
+                                 its engine wave), summed over its launches */
 } amcafe_stats;
 
 int amcafe_domain_create(const amcafe_domain_desc* desc, amcafe_domain** out);
