@@ -58,6 +58,11 @@ WORKLOADS = {
 }
 
 
+def note(msg):
+    """A progress line on stderr (the result line alone goes to stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(stage, nvox, st):
     """Bytes one launch of a stage must move (each array element read or written once)."""
     if stage == 0:   # T write 8 + vState r/w 2 + BuildjID r/w 2 (device layout)
@@ -107,7 +112,10 @@ def batched_vs_single(deck_dir):
     want = [_digest(sim, i) for i in range(len(decks))]
     sim.close()
     ts, sup, same = 0.0, 0, 0
+    note(f"batched: {len(decks)} instances in {tb:.2f} s; now each alone")
     for i, d in enumerate(decks):
+        if i % 16 == 0:
+            note(f"single runs: {i} of {len(decks)}")
         one = amcafe_amd.Sim(d)
         inf = one.info(0)
         t0 = time.perf_counter()
@@ -222,7 +230,7 @@ def main():
     ap.add_argument("--start-step", type=int, default=150,
                     help="step the run is advanced to (untimed) before the warm-up and timed steps")
     ap.add_argument("--workload", default="c1_raster", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=16, help="oracle steps of the CPU baseline (about 14 s on c1)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-full-run", action="store_true", help="skip the whole-run measurement")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-single comparison")
@@ -323,8 +331,10 @@ def main():
     cyc_per_event = tot["loop_cycles"] / tot["n_events"] if tot["n_events"] > 0 else None
     loop_clock_ghz = tot["loop_cycles"] / (stage_ms[4] * 1e6) if stage_ms[4] > 0 else None
 
+    note(f"c1 window timed: {1e3 * wall / max(steps, 1):.3f} ms/step")
     cpu = None
     if snap is not None:
+        note("cpu baseline sample")
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_util import Oracle
         orc = Oracle(deck, crmath=False, bare=True)
@@ -342,10 +352,14 @@ def main():
                          f"the start of the timed window), one thread, {tc:.1f} s; the reference's N-rank MPI "
                          f"figure is not measured"}
 
+    if not args.no_full_run:
+        note("full c1 run")
     whole = None if args.no_full_run else full_run(deck)
     batched = None
     if not args.no_batched:
         batched = batched_vs_single(os.path.join(ROOT, "decks", "c2_batched"))
+    if not args.no_adjoint:
+        note("adjoint cost")
     adjoint = None if args.no_adjoint else adjoint_cost()
     calibration = calibration_run() if args.calibration else None
 
