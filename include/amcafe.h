@@ -102,7 +102,8 @@ int amcafe_analytic_temp_curr(amcafe_domain* d, const amcafe_laser* laser, int* 
  * AnalyticTempCurr: the Gaussian moving source (absorptivity 0.56, radius 50 um, power bmP)
  * integrated over the last 8e-4 s with the 25-point Gauss-Legendre rule, 316L properties by a
  * damped Picard iteration (tolerance 10 K, at most 100 iterations), FP64 on the device.  0 K
- * above the laser plane, 0.99 T0 ahead of the laser in the top layer (layerT).  *all_melt: every
+ * above the laser plane, 0.99 T0 ahead of the laser within the top layer (layerT: the layer
+ * thickness, Grid::layerT, S/TempField.C:103).  *all_melt: every
  * voxel at T >= tL (the end-of-scan test of S/TempField.C:117-126); *n_integrals: source
  * integrals evaluated.  Either may be NULL.  The reference cannot select this model from a deck
  * (its EASM keyword never matches, S/Grid.C:198-199,276).  amcafe_update_voxels after it grows
