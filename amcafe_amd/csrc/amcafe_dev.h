@@ -85,6 +85,8 @@ int amcd_get_rot(amcd_batch* b, int i, double* rot9, int ngrain);
 int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n);
 // every instance's pool in one staged copy (rot9[i] NULL or n[i] = 0: instance i unchanged)
 int amcd_set_pools(amcd_batch* b, const double* const* rot9, const int* n);
+// graph launches of the step so far (all batches of the process)
+long long amcd_graph_launches(void);
 
 // the time step: all instances of the batch advance together.
 // stage 1: temperature only (AnalyticTempCurr seam)

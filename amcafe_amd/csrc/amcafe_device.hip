@@ -37,6 +37,8 @@
 #include <string>
 #include <vector>
 
+#include <atomic>
+
 #include "amcafe_dev.h"
 #include "ddmath.h"
 #include "gl25.h"
@@ -2606,6 +2608,10 @@ __global__ __launch_bounds__(GB) void k_mushy_to_solid(Inst* I, const int* astar
 // ---- per-layer helpers ---------------------------------------------------------------
 // After a host upload of vState: the "T >= tL" flag (bit 7) follows the device's current
 // temperature field, so SetLiquid3 still sees the caller's melt pool.
+// Fresh output records inside a captured step (a memset node is not used in graphs).
+__global__ void k_clear_outs(unsigned* o, int nwords) {
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < nwords; q += gridDim.x * blockDim.x) o[q] = 0u;
+}
 __global__ void k_rehot(uint8_t* vs, const double* T, int N, double tL) {
   size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= (size_t)N) return;
@@ -2773,6 +2779,15 @@ struct amcd_batch {
   double* h_pool = nullptr;  // pinned
   std::vector<size_t> pool_off;
   hipEvent_t pool_ev = nullptr;  // the last staging copy (the staging area is rewritten after it)
+  // The step as HIP graphs (AMCAFE_GRAPH=0 launches the stages one by one): the stages up to
+  // the capture loop's outputs, per temperature form and capture tier, and the finishing pair;
+  // each is captured again when the block counts it was captured with change (capacity growth)
+  bool graphs = true;
+  hipGraphExec_t gstep[6] = {};
+  long long gstep_key[6] = {};
+  hipGraphExec_t gfin = nullptr;
+  long long gfin_key = -1;
+  std::vector<int> astart_up, lstart_up;  // the block lists the device holds
 };
 
 static thread_local std::string g_dev_err;
@@ -2841,6 +2856,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   amcd_batch* b = new amcd_batch();
   b->n = n;
   b->h.resize(n);
+  if (const char* g = getenv("AMCAFE_GRAPH")) b->graphs = atoi(g) != 0;
   if (const char* mt = getenv("AMCAFE_CAPTURE_TIER_MIN")) {
     b->min_tier = std::max(0, std::min(5, atoi(mt)));  // 5: the memory tier with the bitmap in memory
     b->tier = std::max(b->tier, b->min_tier);
@@ -2973,6 +2989,9 @@ void amcd_destroy(amcd_batch* b) {  // best effort: errors while tearing down ar
     if (b->ev[q]) (void)hipEventDestroy(b->ev[q]);
   if (b->h_steps) (void)hipHostFree(b->h_steps);
   if (b->h_outs) (void)hipHostFree(b->h_outs);
+  for (auto& g : b->gstep)
+    if (g) (void)hipGraphExecDestroy(g);
+  if (b->gfin) (void)hipGraphExecDestroy(b->gfin);
   if (b->h_pool) (void)hipHostFree(b->h_pool);
   if (b->d_pool) (void)hipFree(b->d_pool);
   if (b->pool_ev) (void)hipEventDestroy(b->pool_ev);
@@ -3278,19 +3297,29 @@ static int read_outs(amcd_batch* b, amcd_out* outs) {
   return 0;
 }
 
-// After the scan the active counts are known; the active-cell kernels are sized by
-// the capacity (idle threads exit), so no host round trip is needed mid-step.
-static int launch_active(amcd_batch* b) {
+// The active-cell block lists, sized by the capacities (idle threads exit, so no host round
+// trip is needed mid-step), uploaded when a capacity has changed.
+static int prep_active(amcd_batch* b) {
   const int n = b->n;
-  // active-cell block list sized by capacity
   b->astart[0] = 0;
   for (int i = 0; i < n; ++i) b->astart[i + 1] = b->astart[i] + (b->h[i].acap + GB - 1) / GB;
-  HIPT(hipMemcpyAsync(b->d_astart, b->astart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
-  hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
-                     b->d_outs);
   b->lstart[0] = 0;
   for (int i = 0; i < n; ++i) b->lstart[i + 1] = b->lstart[i] + std::min(2048, (b->h[i].acap + 3) / 4);
-  HIPT(hipMemcpyAsync(b->d_lstart, b->lstart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+  if (b->astart != b->astart_up) {
+    HIPT(hipMemcpyAsync(b->d_astart, b->astart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+    b->astart_up = b->astart;
+  }
+  if (b->lstart != b->lstart_up) {
+    HIPT(hipMemcpyAsync(b->d_lstart, b->lstart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+    b->lstart_up = b->lstart;
+  }
+  // the vectors stay unchanged until the next capacity change, so the copies may read them late
+  return 0;
+}
+static int launch_active_kernels(amcd_batch* b) {
+  const int n = b->n;
+  hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
+                     b->d_outs);
   hipLaunchKernelGGL(k_grow_lists, dim3(b->lstart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_lstart, n);
   tmark(b, 4);
   if (launch_event_loop(b)) return -1;
@@ -3298,6 +3327,7 @@ static int launch_active(amcd_batch* b) {
   HIPT(hipGetLastError());
   return 0;
 }
+static int launch_active(amcd_batch* b) { return prep_active(b) || launch_active_kernels(b); }
 
 static int launch_finish(amcd_batch* b) {
   const int n = b->n;
@@ -3395,7 +3425,108 @@ static void pick_tier(amcd_batch* b, const amcd_out* outs) {
   b->tier = std::max(t, b->min_tier);
 }
 
+// Capture of the step's stages into an executable graph: with `step` the step record upload,
+// the output clear, the temperature and state passes, the active-list and candidate-list
+// passes, the capture loop of the current tier and the outputs' download; otherwise the
+// scatter and mushy->solid pair.
+static int capture_graph(amcd_batch* b, bool step, bool fused, hipGraphExec_t* out) {
+  static_assert(sizeof(amcd_out) % 4 == 0, "outputs cleared as words");
+  const int n = b->n;
+  hipGraph_t g = nullptr;
+  HIPT(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  if (step) {
+    const int nw = (int)(sizeof(amcd_out) / 4) * n;
+    rc = hipMemcpyAsync(b->d_steps, b->h_steps, sizeof(amcd_step) * n, hipMemcpyHostToDevice, b->stream) !=
+         hipSuccess;
+    if (!rc) {
+      hipLaunchKernelGGL(k_clear_outs, dim3((nw + 255) / 256), dim3(256), 0, b->stream, (unsigned*)b->d_outs, nw);
+      rc = launch_update(b, fused) || launch_active_kernels(b);
+    }
+    if (!rc)
+      rc = hipMemcpyAsync(b->h_outs, b->d_outs, sizeof(amcd_out) * n, hipMemcpyDeviceToHost, b->stream) !=
+           hipSuccess;
+  } else {
+    rc = launch_finish(b);
+  }
+  const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (g_dev_err.empty() || e != hipSuccess) g_dev_err = std::string("step graph capture: ") + hipGetErrorString(e);
+    return -1;
+  }
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    g_dev_err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+    return -1;
+  }
+  return 0;
+}
+static std::atomic<long long> g_graph_launches{0};
+long long amcd_graph_launches(void) { return g_graph_launches.load(); }
+// graph of the step's stages for the current tier, captured again when the block counts change
+static int launch_step_graph(amcd_batch* b) {
+  const int n = b->n, t = std::min(b->tier, 5);
+  const long long key = ((long long)b->astart[n] << 32) | (unsigned)b->lstart[n];
+  hipGraphExec_t& g = b->gstep[t];
+  if (g && b->gstep_key[t] != key) {
+    HIPT(hipGraphExecDestroy(g));
+    g = nullptr;
+  }
+  if (!g) {
+    if (capture_graph(b, true, true, &g)) return -1;
+    b->gstep_key[t] = key;
+  }
+  HIPT(hipGraphLaunch(g, b->stream));
+  g_graph_launches++;
+  return 0;
+}
+static int launch_finish_graph(amcd_batch* b) {
+  const long long key = b->astart[b->n];
+  if (b->gfin && b->gfin_key != key) {
+    HIPT(hipGraphExecDestroy(b->gfin));
+    b->gfin = nullptr;
+  }
+  if (!b->gfin) {
+    if (capture_graph(b, false, false, &b->gfin)) return -1;
+    b->gfin_key = key;
+  }
+  HIPT(hipGraphLaunch(b->gfin, b->stream));
+  g_graph_launches++;
+  return 0;
+}
+
+// The step with its stages in graphs: one graph launch up to the capture loop's outputs (which
+// it copies to the pinned output records), then, when every instance has finished its loop,
+// one for the finishing pair.  Suspensions (pool refill, larger tier, wide bitmap) and capacity
+// growth take the same host paths as the stage-by-stage form.
+static int update_graph(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
+  memcpy(b->h_steps, steps, sizeof(amcd_step) * b->n);  // read by the graph's upload node
+  for (int attempt = 0; attempt < 8; ++attempt) {
+    if (prep_active(b) || launch_step_graph(b)) return -1;
+    HIPT(hipStreamSynchronize(b->stream));
+    memcpy(outs, b->h_outs, sizeof(amcd_out) * b->n);
+    bool ovf = false;
+    for (int i = 0; i < b->n; ++i) ovf = ovf || outs[i].overflow;
+    if (!ovf) break;
+    if (grow_active(b, outs)) return -1;
+  }
+  if (widen(b, outs) || regrow_tier(b, outs)) return -1;
+  bool pend = false;
+  for (int i = 0; i < b->n; ++i) pend = pend || outs[i].status == 1;
+  if (!pend) {
+    if (launch_finish_graph(b)) return -1;
+    HIPT(hipStreamSynchronize(b->stream));
+    pick_tier(b, outs);
+  }
+  return 0;
+}
+
 static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, bool fused) {
+  // the driver's fused step runs as graphs; per-stage timing needs events between the stages,
+  // and the seam form keeps the output records its temperature call wrote: stage by stage
+  if (b->graphs && !b->timing && fused) return update_graph(b, steps, outs);
   if (put_steps(b, steps)) return -1;
   for (int attempt = 0; attempt < 8; ++attempt) {
     if (fused || attempt > 0) {

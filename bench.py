@@ -370,6 +370,7 @@ def main():
         "config": {"workload": args.workload, "description": WORKLOADS[args.workload], "voxels": nvox,
                    "instances": 1, "setup_s": round(t_setup, 2),
                    "start_step": args.start_step, "fast_forward_s": round(t_ff, 3),
+                   "step_graph": os.environ.get("AMCAFE_GRAPH", "1") != "0",
                    "timed_steps": f"{first_timed}-{first_timed + steps - 1}",
                    "stage_ms_per_step": dict(zip(STAGES, np.round(stage_ms / tsteps, 4).tolist())),
                    "stage_steps": tsteps,
