@@ -3425,10 +3425,11 @@ static void pick_tier(amcd_batch* b, const amcd_out* outs) {
   b->tier = std::max(t, b->min_tier);
 }
 
-// Capture of the step's stages into an executable graph: with `step` the step record upload,
-// the output clear, the temperature and state passes, the active-list and candidate-list
-// passes, the capture loop of the current tier and the outputs' download; otherwise the
-// scatter and mushy->solid pair.
+// Capture of the step's stages into an executable graph: with `step` the output clear, the
+// temperature and state passes, the active-list and candidate-list passes and the capture loop
+// of the current tier; otherwise the scatter and mushy->solid pair.  (The step record's upload
+// and the outputs' download stay plain stream copies around the graph: as graph nodes they
+// made the step slower.)
 static int capture_graph(amcd_batch* b, bool step, bool fused, hipGraphExec_t* out) {
   static_assert(sizeof(amcd_out) % 4 == 0, "outputs cleared as words");
   const int n = b->n;
@@ -3437,15 +3438,8 @@ static int capture_graph(amcd_batch* b, bool step, bool fused, hipGraphExec_t* o
   int rc = 0;
   if (step) {
     const int nw = (int)(sizeof(amcd_out) / 4) * n;
-    rc = hipMemcpyAsync(b->d_steps, b->h_steps, sizeof(amcd_step) * n, hipMemcpyHostToDevice, b->stream) !=
-         hipSuccess;
-    if (!rc) {
-      hipLaunchKernelGGL(k_clear_outs, dim3((nw + 255) / 256), dim3(256), 0, b->stream, (unsigned*)b->d_outs, nw);
-      rc = launch_update(b, fused) || launch_active_kernels(b);
-    }
-    if (!rc)
-      rc = hipMemcpyAsync(b->h_outs, b->d_outs, sizeof(amcd_out) * n, hipMemcpyDeviceToHost, b->stream) !=
-           hipSuccess;
+    hipLaunchKernelGGL(k_clear_outs, dim3((nw + 255) / 256), dim3(256), 0, b->stream, (unsigned*)b->d_outs, nw);
+    rc = launch_update(b, fused) || launch_active_kernels(b);
   } else {
     rc = launch_finish(b);
   }
@@ -3497,16 +3491,14 @@ static int launch_finish_graph(amcd_batch* b) {
   return 0;
 }
 
-// The step with its stages in graphs: one graph launch up to the capture loop's outputs (which
-// it copies to the pinned output records), then, when every instance has finished its loop,
-// one for the finishing pair.  Suspensions (pool refill, larger tier, wide bitmap) and capacity
-// growth take the same host paths as the stage-by-stage form.
+// The step with its stages in graphs: one graph launch up to the capture loop's outputs, then,
+// when every instance has finished its loop, one for the finishing pair.  Suspensions (pool
+// refill, larger tier, wide bitmap) and capacity growth take the same host paths as the
+// stage-by-stage form.
 static int update_graph(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
-  memcpy(b->h_steps, steps, sizeof(amcd_step) * b->n);  // read by the graph's upload node
+  if (put_steps(b, steps)) return -1;
   for (int attempt = 0; attempt < 8; ++attempt) {
-    if (prep_active(b) || launch_step_graph(b)) return -1;
-    HIPT(hipStreamSynchronize(b->stream));
-    memcpy(outs, b->h_outs, sizeof(amcd_out) * b->n);
+    if (prep_active(b) || launch_step_graph(b) || read_outs(b, outs)) return -1;
     bool ovf = false;
     for (int i = 0; i < b->n; ++i) ovf = ovf || outs[i].overflow;
     if (!ovf) break;
