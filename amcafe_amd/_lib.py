@@ -24,6 +24,11 @@ class Stats(ctypes.Structure):
                 ("interface_events", ctypes.c_longlong), ("loop_cycles", ctypes.c_longlong)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("list_blocks", ctypes.c_int), ("tunings", ctypes.c_int), ("growing_cells", ctypes.c_int),
+                ("candidates", ctypes.c_int * 5), ("ms", ctypes.c_float * 5)]
+
+
 class Info(ctypes.Structure):
     _fields_ = [("nX", ctypes.c_int * 3), ("nGrain", ctypes.c_int), ("tInd", ctypes.c_int), ("isp", ctypes.c_int),
                 ("indlayer", ctypes.c_int), ("ilaserLoc", ctypes.c_int), ("nlayerTot", ctypes.c_int),
@@ -63,6 +68,7 @@ SIGNATURES = {
     "amcafe_sim_step": (_I, [_P, ctypes.POINTER(_I)]),
     "amcafe_sim_info": (_I, [_P, _I, ctypes.POINTER(Info)]),
     "amcafe_sim_stats": (_I, [_P, _I, ctypes.POINTER(Stats)]),
+    "amcafe_sim_tuning": (_I, [_P, ctypes.POINTER(Tuning)]),
     "amcafe_sim_get_fields": (_I, [_P, _I, _P, _P, _P, _P, _P, _P, _P]),
     "amcafe_sim_get_grains": (_I, [_P, _I, _P, ctypes.POINTER(_I)]),
     "amcafe_sim_write_hdf5": (_I, [_P, _I, _C]),

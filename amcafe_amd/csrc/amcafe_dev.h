@@ -87,6 +87,9 @@ int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n);
 int amcd_set_pools(amcd_batch* b, const double* const* rot9, const int* n);
 // graph launches of the step so far (all batches of the process)
 long long amcd_graph_launches(void);
+// the chunking the autotuner picked (0: not yet), how often it tuned, the growing-list size of
+// the last tuning and each candidate's time (ms) there; returns 1 once tuned
+int amcd_tuning(amcd_batch* b, int* list_cap, int* n_tunings, int* cells, float* ms, int nms);
 
 // the time step: all instances of the batch advance together.
 // stage 1: temperature only (AnalyticTempCurr seam)

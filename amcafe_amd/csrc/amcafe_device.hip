@@ -2788,6 +2788,15 @@ struct amcd_batch {
   hipGraphExec_t gfin = nullptr;
   long long gfin_key = -1;
   std::vector<int> astart_up, lstart_up;  // the block lists the device holds
+  // Active-list chunking: blocks per instance of the candidate-list pass, picked by timing
+  // the pass (which only rebuilds lists from their inputs) at each candidate, again each time
+  // the largest growing list has doubled (AMCAFE_AUTOTUNE=0 keeps the default)
+  int list_cap = 2048;
+  bool autotune = true, tuned = false;
+  bool tune_pending = false;  // the next step tunes (between its list pass and its capture loop)
+  int tuned_at = 0;  // largest growing list at the last tuning
+  int n_tunings = 0;
+  float tune_ms[8] = {0};
 };
 
 static thread_local std::string g_dev_err;
@@ -2857,6 +2866,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
   b->n = n;
   b->h.resize(n);
   if (const char* g = getenv("AMCAFE_GRAPH")) b->graphs = atoi(g) != 0;
+  if (const char* t = getenv("AMCAFE_AUTOTUNE")) b->autotune = atoi(t) != 0;
   if (const char* mt = getenv("AMCAFE_CAPTURE_TIER_MIN")) {
     b->min_tier = std::max(0, std::min(5, atoi(mt)));  // 5: the memory tier with the bitmap in memory
     b->tier = std::max(b->tier, b->min_tier);
@@ -3304,7 +3314,7 @@ static int prep_active(amcd_batch* b) {
   b->astart[0] = 0;
   for (int i = 0; i < n; ++i) b->astart[i + 1] = b->astart[i] + (b->h[i].acap + GB - 1) / GB;
   b->lstart[0] = 0;
-  for (int i = 0; i < n; ++i) b->lstart[i + 1] = b->lstart[i] + std::min(2048, (b->h[i].acap + 3) / 4);
+  for (int i = 0; i < n; ++i) b->lstart[i + 1] = b->lstart[i] + std::min(b->list_cap, (b->h[i].acap + 3) / 4);
   if (b->astart != b->astart_up) {
     HIPT(hipMemcpyAsync(b->d_astart, b->astart.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
     b->astart_up = b->astart;
@@ -3316,11 +3326,16 @@ static int prep_active(amcd_batch* b) {
   // the vectors stay unchanged until the next capacity change, so the copies may read them late
   return 0;
 }
+static int autotune_lists(amcd_batch* b);
 static int launch_active_kernels(amcd_batch* b) {
   const int n = b->n;
   hipLaunchKernelGGL(k_active_setup, dim3(b->astart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_astart, n,
                      b->d_outs);
   hipLaunchKernelGGL(k_grow_lists, dim3(b->lstart[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_lstart, n);
+  if (b->tune_pending && !b->timing) {  // stage by stage only: the graphs never tune
+    b->tune_pending = false;
+    if (autotune_lists(b)) return -1;
+  }
   tmark(b, 4);
   if (launch_event_loop(b)) return -1;
   tmark(b, 5);
@@ -3491,6 +3506,61 @@ static int launch_finish_graph(amcd_batch* b) {
   return 0;
 }
 
+// Candidate chunkings of the candidate-list pass (blocks per instance).  The pass rebuilds every
+// live slot's list from inputs it does not write, so running it again right after itself, before
+// the capture loop, changes nothing; each candidate is timed three times after a warm-up on the
+// step's interface.  The device's block list is left with the last candidate's and uploaded
+// again for the chosen one at the next step.
+static const int kListCaps[5] = {256, 512, 1024, 2048, 4096};
+static int autotune_lists(amcd_batch* b) {
+  const int n = b->n;
+  std::vector<int> ls(n + 1);
+  float best = 1e30f;
+  int pick = b->list_cap;
+  for (int c = 0; c < 5; ++c) {
+    ls[0] = 0;
+    for (int i = 0; i < n; ++i) ls[i + 1] = ls[i] + std::min(kListCaps[c], (b->h[i].acap + 3) / 4);
+    HIPT(hipMemcpyAsync(b->d_lstart, ls.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_grow_lists, dim3(ls[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_lstart, n);
+    HIPT(hipEventRecord(b->ev[14], b->stream));
+    for (int r = 0; r < 3; ++r)
+      hipLaunchKernelGGL(k_grow_lists, dim3(ls[n]), dim3(GB), 0, b->stream, b->d_inst, b->d_lstart, n);
+    HIPT(hipEventRecord(b->ev[15], b->stream));
+    HIPT(hipGetLastError());
+    HIPT(hipEventSynchronize(b->ev[15]));
+    float ms = 0;
+    HIPT(hipEventElapsedTime(&ms, b->ev[14], b->ev[15]));
+    b->tune_ms[c] = ms / 3;
+    if (ms < best) {
+      best = ms;
+      pick = kListCaps[c];
+    }
+  }
+  b->list_cap = pick;
+  b->tuned = true;
+  b->lstart_up.clear();  // the device's list now holds the last candidate's: upload at the next step
+  return 0;
+}
+// after a finished step: the next step tunes once the largest growing list has reached 256
+// cells, and again each time it has doubled since
+static int maybe_autotune(amcd_batch* b, const amcd_out* outs) {
+  if (!b->autotune) return 0;
+  int mx = 0;
+  for (int i = 0; i < b->n; ++i) mx = std::max(mx, outs[i].n_slots);
+  if (mx < 256 || mx < 2 * b->tuned_at) return 0;
+  b->tuned_at = mx;
+  b->n_tunings++;
+  b->tune_pending = true;
+  return 0;
+}
+int amcd_tuning(amcd_batch* b, int* list_cap, int* n_tunings, int* cells, float* ms, int nms) {
+  if (list_cap) *list_cap = b->tuned ? b->list_cap : 0;
+  if (n_tunings) *n_tunings = b->n_tunings;
+  if (cells) *cells = b->tuned_at;
+  for (int c = 0; c < nms && c < 5; ++c) ms[c] = b->tune_ms[c];
+  return b->tuned ? 1 : 0;
+}
+
 // The step with its stages in graphs: one graph launch up to the capture loop's outputs, then,
 // when every instance has finished its loop, one for the finishing pair.  Suspensions (pool
 // refill, larger tier, wide bitmap) and capacity growth take the same host paths as the
@@ -3511,6 +3581,7 @@ static int update_graph(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
     if (launch_finish_graph(b)) return -1;
     HIPT(hipStreamSynchronize(b->stream));
     pick_tier(b, outs);
+    if (maybe_autotune(b, outs)) return -1;
   }
   return 0;
 }
@@ -3518,7 +3589,7 @@ static int update_graph(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
 static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, bool fused) {
   // the driver's fused step runs as graphs; per-stage timing needs events between the stages,
   // and the seam form keeps the output records its temperature call wrote: stage by stage
-  if (b->graphs && !b->timing && fused) return update_graph(b, steps, outs);
+  if (b->graphs && !b->timing && fused && !b->tune_pending) return update_graph(b, steps, outs);
   if (put_steps(b, steps)) return -1;
   for (int attempt = 0; attempt < 8; ++attempt) {
     if (fused || attempt > 0) {
@@ -3544,6 +3615,7 @@ static int update_common(amcd_batch* b, const amcd_step* steps, amcd_out* outs, 
     HIPT(hipStreamSynchronize(b->stream));
     collect_timing(b);
     pick_tier(b, outs);
+    if (maybe_autotune(b, outs)) return -1;
   }
   return 0;
 }

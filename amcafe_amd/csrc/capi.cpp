@@ -564,6 +564,14 @@ extern "C" int amcafe_sim_info(amcafe_sim* s, int i, amcafe_info* info) {
   return AMCAFE_OK;
 }
 
+extern "C" int amcafe_sim_tuning(amcafe_sim* s, amcafe_tuning* t) {
+  if (!s || !t) return fail(AMCAFE_ERR_ARG, "null sim or output");
+  static const int caps[5] = {256, 512, 1024, 2048, 4096};
+  for (int c = 0; c < 5; ++c) t->candidates[c] = caps[c];
+  amcd_tuning(s->dev, &t->list_blocks, &t->tunings, &t->growing_cells, t->ms, 5);
+  return AMCAFE_OK;
+}
+
 extern "C" int amcafe_sim_stats(amcafe_sim* s, int i, amcafe_stats* st) {
   if (!s || i < 0 || i >= (int)s->runs.size() || !st) return fail(AMCAFE_ERR_ARG, "bad instance");
   to_stats(s->runs[i].last, st);

@@ -8,7 +8,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import DomainDesc, Info, Laser, Stats, check, lib
+from ._lib import DomainDesc, Info, Laser, Stats, Tuning, check, lib
 
 
 def read_deck(path):
@@ -59,6 +59,13 @@ class Sim:
         o = Stats()
         check(lib().amcafe_sim_stats(self.h, i, ctypes.byref(o)))
         return {k: getattr(o, k) for k, _ in Stats._fields_}
+
+    def tuning(self):
+        """The launch autotuner's state: chunking in use, tunings, list size, candidates' times."""
+        t = Tuning()
+        check(lib().amcafe_sim_tuning(self.h, ctypes.byref(t)))
+        return {"list_blocks": t.list_blocks, "tunings": t.tunings, "growing_cells": t.growing_cells,
+                "candidates_ms": {int(c): round(float(m), 4) for c, m in zip(t.candidates, t.ms)}}
 
     def fields(self, i=0, which=("gID", "vState", "TempCurr", "extents", "centroidOct", "BuildjID", "Build")):
         nx, ny, nz = self.info(i)["nX"]

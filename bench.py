@@ -325,6 +325,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_source": tsrc,
                 "kernel": KERNEL_NAMES[dom], "launch_ms": round(launch_ms, 4),
                 "algorithmic_bytes_per_launch": round(per_launch, 1), "window": window + " (HIP events)"}
+    tuning = sim.tuning()
     ev = tot["n_events"] / tsteps
     cap_us_per_event = (stage_ms[4] / tsteps) * 1e3 / ev if ev > 0 else None
     # the loop's own clock (shader cycles between its first decision and its exit, engine wave)
@@ -371,6 +372,7 @@ def main():
                    "instances": 1, "setup_s": round(t_setup, 2),
                    "start_step": args.start_step, "fast_forward_s": round(t_ff, 3),
                    "step_graph": os.environ.get("AMCAFE_GRAPH", "1") != "0",
+                   "autotune": tuning,
                    "timed_steps": f"{first_timed}-{first_timed + steps - 1}",
                    "stage_ms_per_step": dict(zip(STAGES, np.round(stage_ms / tsteps, 4).tolist())),
                    "stage_steps": tsteps,

@@ -137,6 +137,19 @@ void amcafe_sim_destroy(amcafe_sim* s);
 int amcafe_sim_step(amcafe_sim* s, int* n_running);
 int amcafe_sim_info(amcafe_sim* s, int inst, amcafe_info* info);
 int amcafe_sim_stats(amcafe_sim* s, int inst, amcafe_stats* stats);
+/* The launch autotuner's state (no counterpart in the reference): the candidate-list pass's
+ * chunking (blocks per instance) is picked by timing the pass at each candidate inside a step,
+ * between the pass and the capture loop (the pass only rebuilds the lists from their inputs),
+ * once the largest growing list reaches 256 cells and again each time it doubles
+ * (AMCAFE_AUTOTUNE=0 keeps the default, 2048).  Results never depend on it. */
+typedef struct {
+  int list_blocks;    /* chunking in use (0: not tuned yet) */
+  int tunings;        /* times the autotuner ran */
+  int growing_cells;  /* largest growing list at the last tuning */
+  int candidates[5];  /* the chunkings tried */
+  float ms[5];        /* each candidate's time per launch at the last tuning */
+} amcafe_tuning;
+int amcafe_sim_tuning(amcafe_sim* s, amcafe_tuning* out);
 /* reference layouts; NULL pointers are skipped; Build is VoxelsCA::Build */
 int amcafe_sim_get_fields(amcafe_sim* s, int inst, int* gID, int* vState, double* TempCurr, double* extents,
                           double* centroidOct, int* BuildjID, int* Build);

@@ -30,6 +30,7 @@ def _lockstep(monkeypatch, names, steps, env=()):
     decks = [read(n) for n in names]
     for k, v in env:
         monkeypatch.setenv(k, v)
+    monkeypatch.setenv("AMCAFE_AUTOTUNE", "0")  # a tuning step runs stage by stage
     monkeypatch.delenv("AMCAFE_GRAPH", raising=False)
     g = amcafe_amd.Sim(decks)
     monkeypatch.setenv("AMCAFE_GRAPH", "0")
