@@ -90,6 +90,16 @@ __device__ unsigned long long g_phase[32];
 #define PHASE_ARRIVE(sh, par, w, flag)
 #define PHASE_LAST(sh, par, NW)
 #endif
+// a per-lane word of what a wave did in the event, for the arrival record (profile builds only)
+#ifdef AMCD_PHASE_PROFILE
+#define PHASE_FLAGS(v) int v = 0
+#define PHASE_FLAG(v, bits) (v) |= (bits)
+#define PHASE_FLAGS_CLEAR(v) (v) = 0
+#else
+#define PHASE_FLAGS(v)
+#define PHASE_FLAG(v, bits)
+#define PHASE_FLAGS_CLEAR(v)
+#endif
 
 // A pointer known to address global memory.  The capture loop reads its array pointers
 // from a descriptor copied to LDS, so the compiler cannot tell their address space and
@@ -1762,8 +1772,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
   x.i1 = sh.r_i1;
   x.dt = 0.0;
   bool apply_first = go == 2;
-  int aflag = 0;  // phase profile only: what this lane did in the event
-  (void)aflag;
+  PHASE_FLAGS(aflag);
   int ks = -1;  // the cell whose list block pb is loaded for its head refresh (-1: none)
   SlotBlock pb = slot_block_none();
   PHASE_INIT(ot);
@@ -1820,7 +1829,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
             // times are non-decreasing along the list: a tie needs the next entry's
             // time to equal t1; then walk the tied block
             if (ok & (tn == t1)) {
-              aflag |= 2;
+              PHASE_FLAG(aflag, 2);
               const int n = gmem(d.gN)[s];
               for (int q = pos + 1; q < n; ++q) {
                 const int tq = gmem(d.cTgt)[c - pos + q];
@@ -1845,7 +1854,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
       if (lane == 0) sh.sb[par][w] = b;
       PHASE(ot, 7, tid == 64);
       PHASE_ARRIVE(sh, par, w, aflag);
-      aflag = 0;
+      PHASE_FLAGS_CLEAR(aflag);
       __syncthreads();
       PHASE(ot, 8, tid == 64);
       // ---- DECIDE
@@ -1888,7 +1897,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
     // the cell created by the previous event (its best was evaluated by the engine)
     const NewCell& nc = sh.mb[par];
     if (nc.valid && nc.slot % NO == mo) {
-      aflag |= 4;
+      PHASE_FLAG(aflag, 4);
       const int kk = nc.slot / NO;
       double e = nc.E;
       if (!x.nuc) {
@@ -1926,7 +1935,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
         const int s = mo + k * NO;
         if (tgt1[k] == i1) {
           const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
-          aflag |= (HC && k == ks) ? 8 : 1;
+          PHASE_FLAG(aflag, (HC && k == ks) ? 8 : 1);
           const SlotHead h = (HC && k == ks) ? slot_head_pre(d, dead, s, pos + 1, i1, pb)
                                              : slot_head(d, dead, s, pos + 1, i1);
           tgt1[k] = h.tgt1;
@@ -1941,7 +1950,7 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
           lo[k] = hi[k] = fInf;
           if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
         } else {
-          aflag |= 16;
+          PHASE_FLAG(aflag, 16);
           tau_bounds(HC ? hc[s] : gmem(d.cD)[c1[k]], E[k], V[k], T, lo[k], hi[k]);
         }
       }
@@ -2267,14 +2276,13 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     bool apply_first = go == 2;
     EnginePrefetch pf;
     bool have_pf = false;  // the event's inputs were prefetched with its decision
-    int eflag = 0;  // phase profile only: what the engine did in the event
-    (void)eflag;
+    PHASE_FLAGS(eflag);
     PHASE_INIT(et);
     while (go > 0) {
       if (!apply_first) {
         PHASE(et, 0, tid == 0);
         PHASE_ARRIVE(sh, par, 0, eflag);
-        eflag = 0;
+        PHASE_FLAGS_CLEAR(eflag);
         __syncthreads();
         PHASE(et, 1, tid == 0);
         PHASE_LAST(sh, par, NW);
@@ -2360,7 +2368,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
           vjs = gmem(d.gV)[sw];
         }
         if (x.nuc) {  // S/VoxelsCA.C:486-506
-          eflag |= 8;
+          PHASE_FLAG(eflag, 8);
           const auto P = gmem(d.pool) + 9 * (size_t)pool_used;
           for (int q = 0; q < 9; ++q) R[q] = P[q];
           G = nGrain + 1;
@@ -2377,9 +2385,9 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
             cen[1] = rc.y;
             cen[2] = rc.z;
             E0 = rc.w;
-            eflag |= 1;
+            PHASE_FLAG(eflag, 1);
           } else {
-            eflag |= 2;
+            PHASE_FLAG(eflag, 2);
             recentre(d, vjs, v1, c0, R, fcap, cen, &E0);
           }
           captures++;
@@ -2403,7 +2411,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
         const bool liq = live_target(dead, tn, i1);
         const unsigned long long bal = __ballot(liq);
         if (cold && bal) {
-          eflag |= 4;
+          PHASE_FLAG(eflag, 4);
           const int slot = nslots;
           const size_t cb = (size_t)slot * AMCD_NB;
           const double V0 = growth_velocity(p, T1);
@@ -2779,9 +2787,9 @@ struct amcd_batch {
   double* h_pool = nullptr;  // pinned
   std::vector<size_t> pool_off;
   hipEvent_t pool_ev = nullptr;  // the last staging copy (the staging area is rewritten after it)
-  // The step as HIP graphs (AMCAFE_GRAPH=0 launches the stages one by one): the stages up to
-  // the capture loop's outputs, per temperature form and capture tier, and the finishing pair;
-  // each is captured again when the block counts it was captured with change (capacity growth)
+  // The driver's step as HIP graphs (AMCAFE_GRAPH=0 launches the stages one by one): the
+  // stages up to the capture loop, per capture tier, and the finishing pair; each is captured
+  // again when the block counts it was captured with change (capacity growth)
   bool graphs = true;
   hipGraphExec_t gstep[6] = {};
   long long gstep_key[6] = {};
