@@ -56,8 +56,11 @@ def _lockstep(monkeypatch, names, steps, env=()):
     return launched
 
 
-def test_graph_step_equals_stage_by_stage(monkeypatch):
-    n = _lockstep(monkeypatch, ["t_small.in"], 120)
+@pytest.mark.parametrize("tier", ["0", "4", "5"])
+def test_graph_step_equals_stage_by_stage(monkeypatch, tier):
+    """Register tiers from the smallest up, the memory-resident tier, and the memory-resident
+    tier with the liveness bitmap in memory (AMCAFE_CAPTURE_TIER_MIN floors the tier)."""
+    n = _lockstep(monkeypatch, ["t_small.in"], 120, env=(("AMCAFE_CAPTURE_TIER_MIN", tier),))
     assert n >= 2 * 120  # the step graph and the finishing graph, every step
 
 
