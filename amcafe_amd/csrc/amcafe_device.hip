@@ -265,13 +265,19 @@ __device__ static inline void decode(const Inst& d, int v, int& i, int& j, int& 
 //    is outside the ellipsoid without the divisions (RN is monotone, so the rounded
 //    quotient is >= 1 and so is the rounded sum of squares);
 //  * while tvalid holds, the previous T of a voxel is known from its hot flag and the
-//    previous laser plane, so T, the flag byte and BuildjID are stored only where they change.
-#define TV 16  // voxels per thread (one 16-byte load of their state bytes)
+//    previous laser plane, so T, the flag byte and BuildjID are stored only where they change;
+//  * a run of voxels whose row segment lies farther from the laser than the melt pool's
+//    extent plus two voxels (in the unrotated frame: the rotation about the plate centre
+//    preserves distances up to rounding far below that margin) cannot reach tL, so in the
+//    planes the pool can reach it changes nothing unless one of its voxels was hot.
+#define TV 16  // voxels per thread (their state bytes in 16-byte loads)
 __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, int ninst, const amcd_step* S,
                                                     amcd_out* O, int setliquid) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
-  const amcd_step& st = S[ii];
+  // The descriptor and the step record are copied: read through references, every field would
+  // be read again from memory after each store of the voxel loop (a store could alias them).
+  const Inst d = I[ii];
+  const amcd_step st = S[ii];
   if (st.skip) return;
   const int v0 = ((blockIdx.x - bstart[ii]) * GB + threadIdx.x) * TV;
   const amcd_params& p = d.p;
@@ -281,16 +287,26 @@ __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, 
     d.ls->zhot1 = min(p.nz - 1, (int)floor(st.lz / p.dz - 0.5) + 1);
   }
   bool anyhot = false;
+  // the three temperatures as values (selected through their struct fields, the compiler
+  // would keep the struct in scratch memory and load through the selected address)
+  const double T0 = p.T0, tS = p.tS, tL = p.tL, xi = p.xi;
   if (v0 < d.N) {
-    const bool valid = d.ls->tvalid != 0 && p.tS < p.tL && p.T0 < p.tL;
+    const bool valid = d.ls->tvalid != 0 && tS < tL && T0 < tL;
     const double lzp = d.ls->lz_prev;
     const double e1 = 1 + 0x1p-40;
     const double lim1 = p.a1[1] * e1, lim2 = p.a1[2] * e1, lim3 = p.a1[3] * e1, lim4 = p.a1[4] * e1,
                  lim5 = p.a1[5] * e1;
     uint8_t sv[TV];
     if (v0 + TV <= d.N) {
-      const uint4 w = *reinterpret_cast<const uint4*>(d.vs + v0);
-      const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+      unsigned ws[TV / 4];
+#pragma unroll
+      for (int c = 0; c < TV / 16; ++c) {  // 16-byte loads of the run's state bytes
+        const uint4 w = *reinterpret_cast<const uint4*>(d.vs + v0 + 16 * c);
+        ws[4 * c] = w.x;
+        ws[4 * c + 1] = w.y;
+        ws[4 * c + 2] = w.z;
+        ws[4 * c + 3] = w.w;
+      }
 #pragma unroll
       for (int q = 0; q < TV; ++q) sv[q] = (uint8_t)(ws[q >> 2] >> (8 * (q & 3)));
     } else {
@@ -309,12 +325,23 @@ __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, 
     // and no state, grain or melt flag is written (below the laser plane the flag is
     // cleared when T < T0, so tS < T0 rules the shortcut out there).
     bool quiet = false;
-    if (valid && v0 + TV <= d.N && (k < kh0 || k > kh1) && (v0 + TV - 1) / d.nxy == k) {
+    if (valid && v0 + TV <= d.N && (v0 + TV - 1) / d.nxy == k) {
       unsigned busy = 0;
 #pragma unroll
       for (int q = 0; q < TV; ++q) busy |= sv[q] & 0xbc;
       const double zp = ((double)k + .5) * p.dz;
-      quiet = busy == 0 && (zp > st.lz) == (zp > lzp) && ((zp > st.lz) || !(p.tS < p.T0));
+      bool cold = k < kh0 || k > kh1 || zp > st.lz;  // no geometry reaches these voxels
+      if (!cold && i + TV <= p.nx) {  // a run inside one row: its distance to the laser
+        const double Lx = p.hx + st.cg * (st.lx - p.hx) - st.sg * (st.ly - p.hy);
+        const double Ly = p.hy + st.sg * (st.lx - p.hx) + st.cg * (st.ly - p.hy);
+        const double xa = ((double)i + .5) * p.dx, xb = ((double)(i + TV - 1) + .5) * p.dx;
+        const double yc = ((double)j + .5) * p.dy;
+        const double ex = fmax(0.0, fmax(xa - Lx, Lx - xb)), ey = yc - Ly;
+        const double A = fmax(p.bmDX0, p.a1[3]), B = fmax(p.a1[1], p.a1[4]);
+        const double R = sqrt(A * A + B * B) + 2.0 * fmax(p.dx, p.dy);
+        cold = ex * ex + ey * ey > R * R;
+      }
+      quiet = busy == 0 && cold && (zp > st.lz) == (zp > lzp) && ((zp > st.lz) || !(tS < T0));
     }
     if (!quiet)
 #pragma unroll
@@ -324,34 +351,34 @@ __global__ __launch_bounds__(GB) void k_temperature(Inst* I, const int* bstart, 
       const double x0 = ((double)i + .5) * p.dx;
       const double y0 = ((double)j + .5) * p.dy;
       const double zp = ((double)k + .5) * p.dz;
-      double T = p.T0;
+      double T = T0;
       if (k < kh0 || k > kh1) {
-        if (!(zp > st.lz)) T = p.tS;
+        if (!(zp > st.lz)) T = tS;
       } else if (!(zp > st.lz)) {
         const double xp = st.cg * (x0 - p.hx) + st.sg * (y0 - p.hy) + p.hx;
         const double yp = (-st.sg) * (x0 - p.hx) + st.cg * (y0 - p.hy) + p.hy;
         const double r0 = xp - st.lx, r1 = yp - st.ly, r2 = zp - st.lz;
-        T = p.tS;
+        T = tS;
         if (st.dirp * r0 > 0) {
           if (fabs(r0) < p.bmDX0 && !(fabs(r1) > lim1) && !(fabs(r2) > lim2)) {
             const double q1 = r1 / p.a1[1], q2 = r2 / p.a1[2];
             const double dsq = q1 * q1 + q2 * q2;
-            if (dsq < 1.0) T = p.xi;
+            if (dsq < 1.0) T = xi;
           }
         } else if (!(fabs(r0) > lim3) && !(fabs(r1) > lim4) && !(fabs(r2) > lim5)) {
           const double q0 = r0 / p.a1[3], q1 = r1 / p.a1[4], q2 = r2 / p.a1[5];
           const double dsq = q0 * q0 + q1 * q1 + q2 * q2;
-          if (dsq < 1.0) T = p.xi;
+          if (dsq < 1.0) T = xi;
         }
       }
       const uint8_t s0 = sv[q];
-      const bool hot = T >= p.tL;
-      const double Told = (s0 & 0x80) ? p.xi : (zp > lzp ? p.T0 : p.tS);
+      const bool hot = T >= tL;
+      const double Told = (s0 & 0x80) ? xi : (zp > lzp ? T0 : tS);
       if (!valid || T != Told) d.T[v] = T;
       if (!(zp > st.lz)) {
-        if (T > p.tL) {
+        if (T > tL) {
           if (d.bj[v] != 1) d.bj[v] = 1;
-        } else if (T < p.T0) {
+        } else if (T < T0) {
           d.bj[v] = 0;
         }
       }
@@ -463,7 +490,7 @@ __device__ static inline double easm_temp(const double L[3], const double R[3], 
 __global__ __launch_bounds__(GB) void k_easm(Inst* I, const int* bstart, int ninst, const amcd_step* S, amcd_out* O,
                                              double Q, double layerT, unsigned long long* n_evals) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   const amcd_step& st = S[ii];
   if (st.skip) return;
   const int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
@@ -501,7 +528,7 @@ __global__ __launch_bounds__(GB) void k_easm(Inst* I, const int* bstart, int nin
 // SetLiquid3 (S/VoxelsCA.C:1098-1116) when the temperature kernel ran without it.
 __global__ __launch_bounds__(GB) void k_setliquid(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   if (S[ii].skip) return;
   int v = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
   if (v >= d.N) return;
@@ -523,7 +550,7 @@ __global__ __launch_bounds__(GB) void k_setliquid(Inst* I, const int* bstart, in
 // (3 -> 2) never changes the predicate (some neighbour == 1) of any other voxel.
 __global__ __launch_bounds__(GB) void k_solid_to_mushy(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   if (S[ii].skip) return;
   const int nx = d.p.nx, ny = d.p.ny, nz = d.p.nz;
   const int b0 = (blockIdx.x - bstart[ii]) * SB;
@@ -650,7 +677,7 @@ __device__ static bool hot_voxel_active(const Inst& d, int v) {
 __global__ __launch_bounds__(256) void k_active_flags(Inst* I, const int* bstart, int ninst, const amcd_step* S) {
   constexpr int PT = ZB / 64;
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   if (S[ii].skip) return;
   const int blk = (blockIdx.x - bstart[ii]) * 4 + (int)(threadIdx.x >> 6);
   if (blk >= d.nblkZ) return;  // whole waves leave: nothing below synchronises the block
@@ -889,7 +916,7 @@ __global__ __launch_bounds__(256) void k_compact(Inst* I, const int* bstart, int
   // one wave per block of ZB voxels (four per workgroup), 16 consecutive voxels per thread
   constexpr int PT = ZB / 64;
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   if (d.ls->done) return;
   const int blk = (blockIdx.x - bstart[ii]) * 4 + (int)(threadIdx.x >> 6);
   if (blk >= d.nblkZ) return;
@@ -1019,7 +1046,7 @@ __device__ static inline bool cand_less(double Da, double Ra, int ka, double Db,
 
 __global__ __launch_bounds__(GB) void k_active_setup(Inst* I, const int* bstart, int ninst, amcd_out* O) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   if (d.ls->done) return;
   int a = (blockIdx.x - bstart[ii]) * GB + threadIdx.x;
   int Na = d.ls->Na;
@@ -1221,7 +1248,7 @@ __device__ static inline void grow_list(const Inst& d, int slot, int lane) {
 // every wave of an instance's blocks takes slots wave, wave + (its waves), ...
 __global__ __launch_bounds__(GB) void k_grow_lists(Inst* I, const int* bstart, int ninst) {
   int ii = find_inst(bstart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   if (d.ls->done) return;
   const int lane = threadIdx.x & 63;
   const int nw = (bstart[ii + 1] - bstart[ii]) * (GB / 64);
@@ -2569,7 +2596,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
 // Active arrays back to the grid (S/VoxelsCA.C:633-643).
 __global__ __launch_bounds__(GB) void k_scatter(Inst* I, const int* astart, int ninst) {
   int ii = find_inst(astart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   int a = (blockIdx.x - astart[ii]) * GB + threadIdx.x;
   if (d.ls->pending || a >= d.ls->Na || a >= d.acap) return;
   int v = d.aVox[a];
@@ -2587,7 +2614,7 @@ __global__ __launch_bounds__(GB) void k_scatter(Inst* I, const int* astart, int 
 // Also clears the voxel -> active map for the next step.
 __global__ __launch_bounds__(GB) void k_mushy_to_solid(Inst* I, const int* astart, int ninst) {
   int ii = find_inst(astart, ninst, blockIdx.x);
-  const Inst& d = I[ii];
+  const Inst d = I[ii];  // a copy: stores in the loops cannot alias it
   int a = (blockIdx.x - astart[ii]) * GB + threadIdx.x;
   if (d.ls->pending || a >= d.ls->Na || a >= d.acap) return;
   int v = d.aVox[a];
