@@ -1738,8 +1738,11 @@ struct HeadCache {
   static constexpr bool on = K > 0 && K <= 8;
 };
 template <int NT, int K>
-__device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigned* dead, double* hc, int go,
+__device__ static void owner_loop(const Inst& d_lds, EngineShared& sh, const unsigned* dead, double* hc, int go,
                                   double T, long long& visits_out, long long& evals_out) {
+  // The descriptor's fields in registers: read through the LDS copy, each would be read again
+  // after every store to LDS (head cache, bests, liveness), which might alias it.
+  const Inst d = d_lds;
   constexpr bool HC = HeadCache<K>::on;
   constexpr int HN = (NT - 64) * K;  // head-cache slots (one per cell of the tier)
   // work counters are kept in registers and handed back once at the end (a reference
@@ -2005,9 +2008,10 @@ __device__ static void owner_loop(const Inst& d, EngineShared& sh, const unsigne
 // memory and every event streams the lane's cells (slots me, me + NO, ...).  Same
 // decisions, same arithmetic, same order of additions as the register tiers.
 template <int NT>
-__device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst& d, EngineShared& sh,
+__device__ static __attribute__((always_inline)) void owner_loop_mem(const Inst& d_lds, EngineShared& sh,
                                                                       const unsigned* dead, int go, double T,
                                                                       long long& visits_out, long long& evals_out) {
+  const Inst d = d_lds;  // in registers, as in owner_loop
   long long visits = 0, evals = 0;
   constexpr int NO = NT - 64;
   constexpr int NW = NT / 64;
