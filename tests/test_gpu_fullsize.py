@@ -5,6 +5,8 @@ first step:
     sample of plate voxels and the first powder layer's labels at voxel corners on every layer
     voxel, against a nearest-site search over the oracle's own site draws; states, the
     renumbering of the used sites and both orientation tables;
+  * c1's first 12 steps from its initial state (the laser still off the plate), field for field
+    against both oracles;
   * c1 handed to both oracles after its first layer change (the laser one layer up, the grid
     turned by 90 deg, a fresh powder layer), twice: two steps after the change, and 60 steps
     into the new layer with the melt pool established; the next two steps field for field;
@@ -66,15 +68,15 @@ def test_full_size_initial_state(deck_name):
     assert np.array_equal(ct[ng:].ravel(), samples(len(used) + 1, sdl))
 
 
-def _handover(sim, deck, label, min_events):
-    """Two steps of the device against both oracles from the device's current state."""
+def _handover(sim, deck, label, min_events, steps=2):
+    """`steps` steps of the device against both oracles from the device's current state."""
     info = sim.info(0)
     f = sim.fields(0, which=("gID", "vState", "extents", "centroidOct", "BuildjID"))
     cr, gl = Oracle(deck, crmath=True, bare=True), Oracle(deck, crmath=False, bare=True)
     for o in (cr, gl):
         o.load_state(f, sim.grains(0), info)
     events = 0
-    for m in range(2):
+    for m in range(steps):
         sim.step()
         for o in (cr, gl):
             o.temperature()
@@ -89,6 +91,16 @@ def _handover(sim, deck, label, min_events):
         events += sim.stats(0)["n_events"]
     assert np.array_equal(sim.grains(0), cr.grains())
     assert events >= min_events, (label, events)
+
+
+def test_c1_first_steps_from_the_initial_state():
+    """Steps 1-12 of c1 from the device's initial state: the laser still approaches the plate
+    from its grid's overhang (capture events start near step 150, where the mid-run hand-over
+    of test_gpu_parity.py takes over), so these steps check the temperature, state and
+    active-set passes on the full grid."""
+    deck = read("c1_raster.in")
+    sim = amcafe_amd.Sim(deck)
+    _handover(sim, deck, "c1 from step 0", 0, steps=12)
 
 
 def test_c1_handover_after_the_first_layer_change():
