@@ -95,9 +95,9 @@ def _handover(sim, deck, label, min_events, steps=2):
 
 def test_c1_first_steps_from_the_initial_state():
     """Steps 1-12 of c1 from the device's initial state: the laser still approaches the plate
-    from its grid's overhang (capture events start near step 150, where the mid-run hand-over
-    of test_gpu_parity.py takes over), so these steps check the temperature, state and
-    active-set passes on the full grid."""
+    from its grid's overhang, so these steps check the temperature, state and active-set
+    passes on the full grid (capture events come later; test_gpu_parity.py hands c1 over at
+    step 150, in the steady raster)."""
     deck = read("c1_raster.in")
     sim = amcafe_amd.Sim(deck)
     _handover(sim, deck, "c1 from step 0", 0, steps=12)
