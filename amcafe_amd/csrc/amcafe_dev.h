@@ -83,6 +83,11 @@ int amcd_download(amcd_batch* b, int i, int* gid, int* vstate, double* ext, doub
 int amcd_set_rot(amcd_batch* b, int i, const double* rot9, int ngrain);
 int amcd_get_rot(amcd_batch* b, int i, double* rot9, int ngrain);
 int amcd_set_pool(amcd_batch* b, int i, const double* rot9, int n);
+// VoxelsCA::SetBuild on the device (grain ids 1..ngrain); Build = 1 on the first n_built voxels;
+// Build as ints
+int amcd_set_build(amcd_batch* b, int i, int ngrain);
+int amcd_build_init(amcd_batch* b, int i, long long n_built);
+int amcd_get_build(amcd_batch* b, int i, int* out);
 // every instance's pool in one staged copy (rot9[i] NULL or n[i] = 0: instance i unchanged)
 int amcd_set_pools(amcd_batch* b, const double* const* rot9, const int* n);
 // graph launches of the step so far (all batches of the process)

@@ -168,6 +168,7 @@ struct Inst {
   double *cx, *cy, *cz;
   double* T;
   uint8_t* bj;
+  uint8_t* build;  // VoxelsCA::Build (changes only where the driver calls SetBuild)
   int* amap;
   unsigned* zliq;  // per z plane: bit 0 = had liquid at the last active-set pass (or unknown)
   int *bcA, *bcM;  // per-ZB-block counts -> exclusive offsets within chunks of 1024 blocks
@@ -2816,6 +2817,8 @@ struct amcd_batch {
   // entries of 9 doubles), filled from one pinned staging copy per step
   double* d_pool = nullptr;
   double* h_pool = nullptr;  // pinned
+  uint8_t* d_mark = nullptr;  // SetBuild's grain marks (mark_cap bytes)
+  size_t mark_cap = 0;
   std::vector<size_t> pool_off;
   hipEvent_t pool_ev = nullptr;  // the last staging copy (the staging area is rewritten after it)
   // The driver's step as HIP graphs (AMCAFE_GRAPH=0 launches the stages one by one): the
@@ -2947,7 +2950,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     d.nblkT = (d.N + GB * TV - 1) / (GB * TV);
     size_t Nn = (size_t)d.N;
     if (dalloc(b, &d.vs, Nn) || dalloc(b, &d.gid, Nn) || dalloc(b, &d.ext, Nn) || dalloc(b, &d.cx, Nn) ||
-        dalloc(b, &d.cy, Nn) || dalloc(b, &d.cz, Nn) || dalloc(b, &d.T, Nn) || dalloc(b, &d.bj, Nn) ||
+        dalloc(b, &d.cy, Nn) || dalloc(b, &d.cz, Nn) || dalloc(b, &d.T, Nn) || dalloc(b, &d.bj, Nn) || dalloc(b, &d.build, Nn) ||
         dalloc(b, &d.amap, Nn) || dalloc(b, &d.bcA, (size_t)d.nblkZ) || dalloc(b, &d.bcM, (size_t)d.nblkZ) ||
         dalloc(b, &d.chA, (size_t)d.nblkZ / 1024 + 1) || dalloc(b, &d.chM, (size_t)d.nblkZ / 1024 + 1) ||
         dalloc(b, &d.zliq, (size_t)p[i].nz) || dalloc(b, &d.ls, 1)) {
@@ -2968,6 +2971,7 @@ int amcd_create(const amcd_params* p, int n, amcd_batch** out, char* err, int er
     zero(d.cz, 0, Nn * 8);
     zero(d.T, 0, Nn * 8);
     zero(d.bj, 0, Nn);
+    zero(d.build, 0, Nn);
     zero(d.zliq, 0xff, (size_t)p[i].nz * 4);  // unknown: every plane may hold liquid
     zero(d.amap, 0xff, Nn * 4);
     zero(d.ls, 0, sizeof(LoopState));
@@ -3043,6 +3047,7 @@ void amcd_destroy(amcd_batch* b) {  // best effort: errors while tearing down ar
   if (b->gfin) (void)hipGraphExecDestroy(b->gfin);
   if (b->h_pool) (void)hipHostFree(b->h_pool);
   if (b->d_pool) (void)hipFree(b->d_pool);
+  if (b->d_mark) (void)hipFree(b->d_mark);
   if (b->pool_ev) (void)hipEventDestroy(b->pool_ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -3678,6 +3683,62 @@ int amcd_resume(amcd_batch* b, const amcd_step* steps, amcd_out* outs) {
     if (launch_finish(b)) return -1;
     HIPT(hipStreamSynchronize(b->stream));
   }
+  return 0;
+}
+
+// VoxelsCA::SetBuild (S/VoxelsCA.C:76-115) as two rounds of grain-set membership: mark the
+// grains of the flagged voxels (BuildjID == 1, then Build == 1), then flag every voxel of a
+// marked grain as built.  Grain ids lie in 1..ngrain.
+__global__ void k_build_mark(const int* __restrict__ gid, const uint8_t* __restrict__ flag, int N, int ngrain,
+                             uint8_t* __restrict__ mark) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N || flag[v] != 1) return;
+  const int g = gid[v];
+  if (g > 0 && g <= ngrain) mark[g] = 1;
+}
+__global__ void k_build_apply(const int* __restrict__ gid, int N, int ngrain, const uint8_t* __restrict__ mark,
+                              uint8_t* __restrict__ build) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const int g = gid[v];
+  if (g > 0 && g <= ngrain && mark[g]) build[v] = 1;
+}
+int amcd_set_build(amcd_batch* b, int i, int ngrain) {
+  const Inst& d = b->h[i];
+  if (b->mark_cap < (size_t)ngrain + 1) {
+    HIPT(hipStreamSynchronize(b->stream));
+    if (b->d_mark) HIPT(hipFree(b->d_mark));
+    b->d_mark = nullptr;
+    b->mark_cap = 0;
+    const size_t cap = std::max<size_t>(4096, 2 * ((size_t)ngrain + 1));
+    HIPT(hipMalloc(&b->d_mark, cap));
+    b->mark_cap = cap;
+  }
+  uint8_t* mk = b->d_mark;
+  const dim3 grid((d.N + 255) / 256);
+  for (int round = 0; round < 2; ++round) {
+    HIPT(hipMemsetAsync(mk, 0, (size_t)ngrain + 1, b->stream));
+    hipLaunchKernelGGL(k_build_mark, grid, dim3(256), 0, b->stream, d.gid, round == 0 ? d.bj : d.build, d.N,
+                       ngrain, mk);
+    hipLaunchKernelGGL(k_build_apply, grid, dim3(256), 0, b->stream, d.gid, d.N, ngrain, mk, d.build);
+  }
+  HIPT(hipGetLastError());
+  return 0;
+}
+// Build = 1 on the first n_built voxels (the base plate, S/VoxelsCA.C:157-160), 0 elsewhere
+int amcd_build_init(amcd_batch* b, int i, long long n_built) {
+  const Inst& d = b->h[i];
+  HIPT(hipMemsetAsync(d.build, 0, (size_t)d.N, b->stream));
+  if (n_built > 0) HIPT(hipMemsetAsync(d.build, 1, (size_t)std::min<long long>(n_built, d.N), b->stream));
+  HIPT(hipStreamSynchronize(b->stream));
+  return 0;
+}
+int amcd_get_build(amcd_batch* b, int i, int* out) {
+  const Inst& d = b->h[i];
+  std::vector<uint8_t> tmp((size_t)d.N);
+  HIPT(hipStreamSynchronize(b->stream));
+  HIPT(hipMemcpy(tmp.data(), d.build, (size_t)d.N, hipMemcpyDeviceToHost));
+  for (size_t v = 0; v < tmp.size(); ++v) out[v] = tmp[v];
   return 0;
 }
 

@@ -298,7 +298,6 @@ struct Run {
   int Nzh = 0, nlayerTot = 0, TtInd = 0;
   double bp_height = 0;
   bool done = false;
-  std::vector<int> Build;  // VoxelsCA::Build (host; changes only at output steps)
   std::vector<int> out_steps;
   std::string out_prefix;  // output files <prefix><tInd>.h5 when set (S/main1.C:53)
   amcd_out last = {};
@@ -343,8 +342,7 @@ static int init_plate(amcafe_sim* s, int i) {
   s->sa.generate(Ngrain, sdloc, r.cTheta);
   r.nGrain = Ngrain;
   // InitializeVoxels marks every voxel with a grain as built: the whole plate (S/VoxelsCA.C:157-160)
-  r.Build.assign((size_t)g.nvox(), 0);
-  std::fill(r.Build.begin(), r.Build.begin() + (size_t)r.Nzh * g.nX[0] * g.nX[1], 1);
+  if (amcd_build_init(s->dev, i, (long long)r.Nzh * g.nX[0] * g.nX[1])) return dev_fail();
   return upload_rot(s, i);
 }
 
@@ -394,19 +392,19 @@ static int clean_layer(amcafe_sim* s, int i) {
   return upload_rot(s, i);
 }
 
-// VoxelsCA::SetBuild (S/VoxelsCA.C:76-115) as set membership over grain ids: voxels
-// of every grain that has a melted voxel, then of every grain with a Build voxel.
 // VoxelsCA::WriteToHDF1 of instance i's current state
 static int write_run(amcafe_sim* s, int i, const char* path) {
   Run& r = s->runs[i];
   const size_t N = (size_t)r.g.nvox();
-  std::vector<int> gid(N), vs(N);
+  std::vector<int> gid(N), vs(N), build(N);
   std::vector<double> T(N);
-  if (amcd_download(s->dev, i, gid.data(), vs.data(), nullptr, nullptr, nullptr, T.data())) return dev_fail();
+  if (amcd_download(s->dev, i, gid.data(), vs.data(), nullptr, nullptr, nullptr, T.data()) ||
+      amcd_get_build(s->dev, i, build.data()))
+    return dev_fail();
   std::string err;
   const int nX[3] = {r.g.nX[0], r.g.nX[1], r.g.nX[2]};
   const double dX[3] = {r.g.dX[0], r.g.dX[1], r.g.dX[2]};
-  if (write_hdf5(path, nX, dX, gid.data(), vs.data(), r.Build.data(), T.data(), r.cTheta.data(), r.nGrain, err))
+  if (write_hdf5(path, nX, dX, gid.data(), vs.data(), build.data(), T.data(), r.cTheta.data(), r.nGrain, err))
     return fail(AMCAFE_ERR_IO, err);
   return AMCAFE_OK;
 }
@@ -416,23 +414,10 @@ static int write_scheduled(amcafe_sim* s, int i) {
   return write_run(s, i, (r.out_prefix + std::to_string(r.TtInd) + ".h5").c_str());
 }
 
+// VoxelsCA::SetBuild (S/VoxelsCA.C:76-115) as set membership over grain ids, on the device:
+// voxels of every grain that has a melted voxel, then of every grain with a Build voxel.
 static int set_build(amcafe_sim* s, int i) {
-  Run& r = s->runs[i];
-  size_t N = (size_t)r.g.nvox();
-  std::vector<int> gid(N), bj(N);
-  if (amcd_download(s->dev, i, gid.data(), nullptr, nullptr, nullptr, bj.data(), nullptr)) return dev_fail();
-  int maxg = 0;
-  for (size_t j = 0; j < N; ++j) maxg = std::max(maxg, gid[j]);
-  std::vector<char> in1(maxg + 1, 0), in2(maxg + 1, 0);
-  for (size_t j = 0; j < N; ++j)
-    if (bj[j] == 1 && gid[j] > 0) in1[gid[j]] = 1;
-  for (size_t j = 0; j < N; ++j)
-    if (gid[j] > 0 && in1[gid[j]]) r.Build[j] = 1;
-  for (size_t j = 0; j < N; ++j)
-    if (r.Build[j] == 1 && gid[j] > 0) in2[gid[j]] = 1;
-  for (size_t j = 0; j < N; ++j)
-    if (gid[j] > 0 && in2[gid[j]]) r.Build[j] = 1;
-  return AMCAFE_OK;
+  return amcd_set_build(s->dev, i, s->runs[i].nGrain) ? dev_fail() : AMCAFE_OK;
 }
 
 extern "C" int amcafe_sim_create(const char* const* decks, int n, amcafe_sim** out) {
@@ -582,7 +567,7 @@ extern "C" int amcafe_sim_get_fields(amcafe_sim* s, int i, int* gID, int* vState
                                      double* cen, int* bj, int* Build) {
   if (!s || i < 0 || i >= (int)s->runs.size()) return fail(AMCAFE_ERR_ARG, "bad instance");
   if (amcd_download(s->dev, i, gID, vState, ext, cen, bj, T)) return dev_fail();
-  if (Build) memcpy(Build, s->runs[i].Build.data(), sizeof(int) * s->runs[i].Build.size());
+  if (Build && amcd_get_build(s->dev, i, Build)) return dev_fail();
   return AMCAFE_OK;
 }
 
