@@ -1435,14 +1435,26 @@ __device__ static inline void block_entries(const Inst& d, size_t cb, int p, Slo
   }
 }
 // The first live entry among positions p .. p+3 of a loaded block into h: true when
-// found or when the list ends before p (h unchanged), false to go on at p + 4.
+// found or when the list ends before p (h unchanged), false to go on at p + 4.  The four
+// liveness words are read unconditionally (an entry past the list end reads word 0), so
+// that they are requested together: read under each entry's own condition, every read
+// waited for the one before.
 __device__ static inline bool block_pick(const SlotBlock& b, const unsigned* dead, size_t cb, int p, int skip,
                                          SlotHead& h) {
   if (p >= b.n) return true;
+  int tq[4];
+  unsigned wq[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    tq[q] = (p + q < b.n && b.t[q] >= 0) ? b.t[q] : -1;
+    wq[q] = dead[(tq[q] >= 0 ? tq[q] : 0) >> 5];
+  }
   int hit = -1;
 #pragma unroll
-  for (int q = 3; q >= 0; --q)
-    if (p + q < b.n && live_target(dead, b.t[q], skip)) hit = q;
+  for (int q = 3; q >= 0; --q) {
+    const bool live = (tq[q] >= 0) & (tq[q] != skip) & (((wq[q] >> (tq[q] & 31)) & 1u) == 0);
+    hit = live ? q : hit;
+  }
   if (hit < 0) return false;
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -1738,9 +1750,12 @@ template <int K>
 struct HeadCache {
   static constexpr bool on = K > 0 && K <= 8;
 };
+// Kept out of line: inlined into the kernel beside the engine's loop, the register tiers'
+// loops ran about 1% slower per event.
 template <int NT, int K>
-__device__ static void owner_loop(const Inst& d_lds, EngineShared& sh, const unsigned* dead, double* hc, int go,
-                                  double T, long long& visits_out, long long& evals_out) {
+__device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, EngineShared& sh,
+                                                            const unsigned* dead, double* hc, int go, double T,
+                                                            long long& visits_out, long long& evals_out) {
   // The descriptor's fields in registers: read through the LDS copy, each would be read again
   // after every store to LDS (head cache, bests, liveness), which might alias it.
   const Inst d = d_lds;
