@@ -1743,12 +1743,23 @@ __device__ static inline double draw_batch(unsigned int x0, unsigned int cb, uns
   return canonical(x1, lcg_next(x1));
 }
 
-// Tiers of up to eight cells per owner thread keep each cell's head (distance, next
-// distance, ruqs) in LDS and its scan key in a register, so that EVAL reads no global
-// memory unless a tie has to be walked; larger tiers read the head from the lists.
+// The register tiers keep each cell's head distance and next distance in LDS and its scan
+// key in a register, so that EVAL reads no global memory unless a tie has to be walked.  Up
+// to eight cells per owner thread the head's ruqs are in LDS too; the sixteen-cell tier
+// (7,168 cells) has room for two doubles per cell and reads the ruqs from the list, a load
+// issued with the LDS reads and needed only after the divisions.  The memory-resident tier
+// reads the head from the lists.
 template <int K>
 struct HeadCache {
-  static constexpr bool on = K > 0 && K <= 8;
+  static constexpr bool on = K > 0 && K <= 16;
+  static constexpr bool ruqs = K > 0 && K <= 8;
+  static constexpr int words = ruqs ? 3 : 2;  // doubles per cell
+};
+// Liveness bits in LDS: the 16-cells-per-thread tier gives half of them to its head cache, and
+// an interface beyond them runs in the memory-resident tier with the bitmap in memory.
+template <int K, bool GD>
+struct DeadBits {
+  static constexpr int n = GD ? 32 : (K == 16 ? (1 << 18) : AMCD_DEAD_BITS);
 };
 // Kept out of line: inlined into the kernel beside the engine's loop, the register tiers'
 // loops ran about 1% slower per event.
@@ -1759,7 +1770,9 @@ __device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, E
   // The descriptor's fields in registers: read through the LDS copy, each would be read again
   // after every store to LDS (head cache, bests, liveness), which might alias it.
   const Inst d = d_lds;
-  constexpr bool HC = HeadCache<K>::on;
+  constexpr bool HC = HeadCache<K>::on, HR = HeadCache<K>::ruqs;
+  // the refresh's block loaded ahead of its use (its registers spilled in the 16-cell tier)
+  constexpr bool PF = HC && K <= 8;
   constexpr int HN = (NT - 64) * K;  // head-cache slots (one per cell of the tier)
   // work counters are kept in registers and handed back once at the end (a reference
   // incremented inside the loop would live in scratch memory)
@@ -1797,7 +1810,7 @@ __device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, E
         c1[k] = (h.c1 - s * AMCD_NB) | ((gmem(d.gScan)[s] * 32 + h.k1) << 5);
         hc[s] = h.D1;
         hc[HN + s] = h.Dn;
-        hc[2 * HN + s] = h.r1;
+        if constexpr (HR) hc[2 * HN + s] = h.r1;
       }
       if (h.tgt1 >= 0) tau_bounds(h.D1, E[k], V[k], T, lo[k], hi[k]);
       // bring the rest of the list (positions 16..25 of the distances and ruqs, the
@@ -1848,7 +1861,7 @@ __device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, E
           if constexpr (HC) {
             D1 = hc[s];
             Dn = hc[HN + s];
-            rr = hc[2 * HN + s];
+            rr = HR ? hc[2 * HN + s] : gmem(d.cR)[c];
             ky = c1[k] >> 5;
             kb = ky & ~31;
           } else {
@@ -1914,7 +1927,7 @@ __device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, E
     // the cell (first of each thread) whose head is the voxel captured now: the next
     // block of its list is loaded before the growth and the takeover below, and waited
     // for where the refresh of its head uses it
-    if constexpr (HC) {
+    if constexpr (PF) {
       ks = -1;
       int pk = 0;
 #pragma unroll
@@ -1962,7 +1975,7 @@ __device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, E
       if constexpr (HC) {
         hc[nc.slot] = nc.D1;
         hc[HN + nc.slot] = nc.Dn1;
-        hc[2 * HN + nc.slot] = nc.r1;
+        if constexpr (HR) hc[2 * HN + nc.slot] = nc.r1;
       }
       if (ks == kk) ks = -1;  // a block loaded for the slot's previous state
     }
@@ -1981,15 +1994,15 @@ __device__ __attribute__((noinline)) static void owner_loop(const Inst& d_lds, E
         const int s = mo + k * NO;
         if (tgt1[k] == i1) {
           const int pos = HC ? (c1[k] & 31) : c1[k] - s * AMCD_NB;
-          PHASE_FLAG(aflag, (HC && k == ks) ? 8 : 1);
-          const SlotHead h = (HC && k == ks) ? slot_head_pre(d, dead, s, pos + 1, i1, pb)
+          PHASE_FLAG(aflag, (PF && k == ks) ? 8 : 1);
+          const SlotHead h = (PF && k == ks) ? slot_head_pre(d, dead, s, pos + 1, i1, pb)
                                              : slot_head(d, dead, s, pos + 1, i1);
           tgt1[k] = h.tgt1;
           if constexpr (HC) {
             c1[k] = (h.c1 - s * AMCD_NB) | ((((c1[k] >> 5) & ~31) + h.k1) << 5);
             hc[s] = h.D1;
             hc[HN + s] = h.Dn;
-            hc[2 * HN + s] = h.r1;
+            if constexpr (HR) hc[2 * HN + s] = h.r1;
           } else {
             c1[k] = h.c1;
           }
@@ -2232,8 +2245,9 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const int CAP = K > 0 ? (NT - 64) * K : I[blockIdx.x].acap;
   __shared__ EngineShared sh;
   __shared__ int s_go;
-  __shared__ unsigned s_dead[GD ? 1 : AMCD_DEAD_BITS / 32];
-  __shared__ double s_hc[HeadCache<K>::on ? 3 * (NT - 64) * K : 1];
+  constexpr int DB = DeadBits<K, GD>::n;
+  __shared__ unsigned s_dead[DB / 32];
+  __shared__ double s_hc[HeadCache<K>::on ? HeadCache<K>::words * (NT - 64) * K : 1];
   // The descriptor and step record are copied to LDS: every event ends in a barrier, after
   // which the compiler must re-read anything in global memory, and each array access would
   // otherwise wait for its base pointer to come back from global memory first.
@@ -2255,7 +2269,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
   const amcd_step& st = s_step;
   const int Na = ls->Na;
   unsigned* const dead = GD ? d.gdead : s_dead;
-  if (GD || Na <= AMCD_DEAD_BITS) {  // liveness bitmap from the current states
+  if (GD || Na <= DB) {  // liveness bitmap from the current states
     for (int wd = tid; wd < (Na + 31) / 32; wd += NT) {
       unsigned m = 0;
       for (int q = 0; q < 32; ++q) {
@@ -2274,7 +2288,7 @@ __global__ __launch_bounds__(NT) void k_event_loop(Inst* I, const amcd_step* S, 
     sh.fin = 0;
     sh.r_i1 = -1;
     s_go = 1;
-    if (!GD && Na > AMCD_DEAD_BITS) {
+    if (!GD && Na > DB) {
       s_go = -3;  // more interface cells than the LDS bitmap holds: the host relaunches with GD
     } else if (ls->nslots > CAP) {
       s_go = -2;  // more growing cells than this tier holds
