@@ -42,6 +42,18 @@ def test_argument_errors_need_no_device():
     assert L.amcafe_last_error()
     L.amcafe_domain_create.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     assert L.amcafe_domain_create(None, ctypes.byref(out)) == -1
+    # the driver-level queries refuse a null simulation or output before any device call
+    buf = ctypes.create_string_buffer(256)
+    for name in ("amcafe_sim_tuning", "amcafe_sim_stats", "amcafe_sim_info"):
+        f = getattr(L, name)
+        if name == "amcafe_sim_tuning":
+            f.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+            rc = f(None, buf)
+        else:
+            f.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+            rc = f(None, 0, buf)
+        assert rc == -1, name
+        assert L.amcafe_last_error(), name
 
 
 def test_product_path_has_no_cpu_fallback(monkeypatch, tmp_path):
