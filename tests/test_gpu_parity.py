@@ -89,9 +89,9 @@ def test_odd_sized_deck_full_run():
 def test_capture_tiers_full_run(monkeypatch, deck, tier):
     """Small decks run with the capture loop held in a larger tier (AMCAFE_CAPTURE_TIER_MIN):
     2 = eight cells per owner thread with the LDS head cache, 3 = sixteen cells per thread
-    reading heads from the lists, 4 = every cell's state in memory, 5 = the same with the
-    liveness bitmap in memory (interfaces beyond the LDS bitmap).  Every step against the
-    oracle."""
+    with head distances in LDS and the head's ruqs read from the list, 4 = every cell's state
+    in memory, 5 = the same with the liveness bitmap in memory (interfaces beyond the LDS
+    bitmap).  Every step against the oracle."""
     monkeypatch.setenv("AMCAFE_CAPTURE_TIER_MIN", str(tier))
     n = _lockstep(deck, 10**6)
     assert n > 100
@@ -196,6 +196,11 @@ def test_c4_production_lockstep_from_mid_run_state():
     _mid_run_handover("c4_production.in", 120, "c4", 1)
 
 
-def test_c1_full_size_lockstep_from_mid_run_state():
-    """Workload c1 at full size (512x512x128) from the GPU's state after 150 steps."""
-    _mid_run_handover("c1_raster.in", 150, "c1", 100)
+@pytest.mark.parametrize("tier", [None, 3])
+def test_c1_full_size_lockstep_from_mid_run_state(monkeypatch, tier):
+    """Workload c1 at full size (512x512x128) from the GPU's state after 150 steps; with tier 3
+    the capture loop runs throughout in the sixteen-cells-per-thread tier (head distances in
+    LDS, ruqs from the lists) instead of the eight-cell tier c1's ~3,200 growing cells pick."""
+    if tier is not None:
+        monkeypatch.setenv("AMCAFE_CAPTURE_TIER_MIN", str(tier))
+    _mid_run_handover("c1_raster.in", 150, "c1" if tier is None else f"c1 tier {tier}", 100)
